@@ -1,0 +1,559 @@
+// amb_batch.hip -- batched ensemble: many independent realisations (own grid size,
+// model, stencil and Philox seed) advanced together.  No reference counterpart.
+//
+// All fields live back to back in two device buffers (ping-pong), addressed with
+// 64-bit offsets.  A step is one launch per tile-shape group over a flattened
+// work list of (realisation, tile) pairs, so hundreds of small grids fill the
+// 256 CUs in one dispatch instead of one under-filled launch each; kGraphSteps
+// steps (all groups, then the step-counter advance) are captured in one hipGraph
+// and replayed.  The autotuner times every compiled tile shape on every distinct
+// grid size and assigns each size class its fastest shape.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <new>
+#include <utility>
+#include <vector>
+
+#include "amb.h"
+#include "amb_internal.h"
+#include "amb_kernels.h"
+#include "amb_tile.h"
+
+namespace amb {
+
+template <typename T>
+struct BatchDesc {
+    int64_t off;  // element offset of the realisation in the concatenated buffers
+    int ny, nx, nx4, ntx;
+    uint32_t key0, key1;
+    StepCoef<T> k;
+};
+
+template <typename T, int TW, int TH>
+__global__ __launch_bounds__(kThreads) void batch_step_kernel(const BatchDesc<T>* __restrict__ desc,
+                                                              const int2* __restrict__ tiles, const T* in, T* out,
+                                                              const long long* step_base, int step_off)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int2 t = tiles[blockIdx.x];
+    if (t.x < 0) return;  // padding entry of the XCD-ordered list
+    const BatchDesc<T>& d = desc[t.x];
+    const int ty = t.y / d.ntx, tx = t.y - ty * d.ntx;
+    const uint64_t step = (uint64_t)(*step_base) + (uint64_t)step_off;
+    step_tile<T, TW, TH, kNoiseRuntime>(reinterpret_cast<T*>(smem_raw), in + d.off, out + d.off, d.ny, d.nx, d.nx4,
+                                        tx * TW, ty * TH, d.k, d.key0, d.key1, step, nullptr);
+}
+
+template <typename T, int TW, int TH>
+static hipError_t launch_batch_one(const void* desc, const int2* tiles, int ntiles, const void* in, void* out,
+                                   const long long* step, int step_off, hipStream_t s)
+{
+    using L = Lds<T, TW, TH>;
+    static bool raised = false;
+    if (!raised && L::bytes > 65536) {
+        const hipError_t e = hipFuncSetAttribute((const void*)batch_step_kernel<T, TW, TH>,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)L::bytes);
+        if (e != hipSuccess) return e;
+        raised = true;
+    }
+    hipLaunchKernelGGL((batch_step_kernel<T, TW, TH>), dim3(ntiles), dim3(kThreads), L::bytes, s,
+                       static_cast<const BatchDesc<T>*>(desc), tiles, static_cast<const T*>(in), static_cast<T*>(out),
+                       step, step_off);
+    return hipGetLastError();
+}
+
+static hipError_t launch_batch(int precision, int tile, const void* desc, const int2* tiles, int ntiles,
+                               const void* in, void* out, const long long* step, int step_off, hipStream_t s)
+{
+    if (precision == AMB_F32) {
+        switch (tile) {
+        case 0: return launch_batch_one<float, 64, 32>(desc, tiles, ntiles, in, out, step, step_off, s);
+        case 1: return launch_batch_one<float, 64, 16>(desc, tiles, ntiles, in, out, step, step_off, s);
+        case 2: return launch_batch_one<float, 32, 32>(desc, tiles, ntiles, in, out, step, step_off, s);
+        default: return launch_batch_one<float, 64, 64>(desc, tiles, ntiles, in, out, step, step_off, s);
+        }
+    }
+    switch (tile) {
+    case 0: return launch_batch_one<double, 32, 32>(desc, tiles, ntiles, in, out, step, step_off, s);
+    case 1: return launch_batch_one<double, 64, 16>(desc, tiles, ntiles, in, out, step, step_off, s);
+    case 2: return launch_batch_one<double, 32, 16>(desc, tiles, ntiles, in, out, step, step_off, s);
+    default: return launch_batch_one<double, 64, 32>(desc, tiles, ntiles, in, out, step, step_off, s);
+    }
+}
+
+}  // namespace amb
+
+using namespace amb;
+
+namespace {
+
+constexpr int kBatchGraphSteps = 32;
+
+struct Group {
+    int tile = 0;
+    std::vector<int> members;
+    int2* d_tiles = nullptr;
+    int ntiles = 0;
+};
+
+}  // namespace
+
+struct amb_batch {
+    int precision = AMB_F32;
+    size_t esize = 4;
+    hipStream_t stream = nullptr;
+    std::vector<amb_realisation> desc;
+    std::vector<int64_t> off;
+    int64_t total = 0;
+    void* buf[2] = {nullptr, nullptr};
+    int cur = 0;
+    long long* d_step = nullptr;
+    long long* d_step_scratch = nullptr;
+    void* d_desc = nullptr;
+    std::vector<int> tile_of;  // tile index per realisation
+    std::vector<Group> groups;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    double* d_part = nullptr;
+    unsigned long long* d_bad_part = nullptr;
+    double* d_sum = nullptr;
+    unsigned long long* d_bad = nullptr;
+    double* h_sum = nullptr;
+    unsigned long long* h_bad = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+void drop_graphs(amb_batch* b)
+{
+    for (auto& g : b->graph) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+}
+
+void free_groups(amb_batch* b)
+{
+    for (auto& g : b->groups)
+        if (g.d_tiles) (void)hipFree(g.d_tiles);
+    b->groups.clear();
+}
+
+void release(amb_batch* b)
+{
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graphs(b);
+    free_groups(b);
+    for (void* p : {b->buf[0], b->buf[1], (void*)b->d_step, (void*)b->d_step_scratch, b->d_desc, (void*)b->d_part,
+                    (void*)b->d_bad_part, (void*)b->d_sum, (void*)b->d_bad})
+        if (p) (void)hipFree(p);
+    if (b->h_sum) (void)hipHostFree(b->h_sum);
+    if (b->h_bad) (void)hipHostFree(b->h_bad);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+// Tile list of `members` for tile shape `tile`, ordered so that the round-robin
+// workgroup-to-XCD dispatch hands each XCD a contiguous run of tiles (neighbouring
+// tiles share halo rows in that XCD's L2).  Padded to a multiple of 8 with (-1, 0).
+std::vector<int2> tile_list(const amb_batch* b, const std::vector<int>& members, int tile)
+{
+    const TileShape ts = tile_shape(b->precision, tile);
+    std::vector<int2> natural;
+    for (int r : members) {
+        const int ntx = grid_tiles(b->desc[r].nx, ts.tw), nty = grid_tiles(b->desc[r].ny, ts.th);
+        for (int t = 0; t < ntx * nty; ++t) natural.push_back(make_int2(r, t));
+    }
+    const size_t n = natural.size();
+    const size_t padded = (n + 7) / 8 * 8;
+    const size_t chunk = padded / 8;
+    std::vector<int2> out(padded, make_int2(-1, 0));
+    for (size_t i = 0; i < n; ++i) out[(i % chunk) * 8 + i / chunk] = natural[i];
+    return out;
+}
+
+int rebuild_groups(amb_batch* b)
+{
+    drop_graphs(b);
+    free_groups(b);
+    std::map<int, std::vector<int>> by_tile;
+    for (size_t r = 0; r < b->desc.size(); ++r) by_tile[b->tile_of[r]].push_back((int)r);
+    for (auto& kv : by_tile) {
+        Group g;
+        g.tile = kv.first;
+        g.members = kv.second;
+        const std::vector<int2> list = tile_list(b, g.members, g.tile);
+        g.ntiles = (int)list.size();
+        hipError_t e = hipMalloc(&g.d_tiles, list.size() * sizeof(int2));
+        if (e == hipSuccess) e = hipMemcpy(g.d_tiles, list.data(), list.size() * sizeof(int2), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            if (g.d_tiles) (void)hipFree(g.d_tiles);
+            return fail_hip(e, "tile list upload");
+        }
+        b->groups.push_back(std::move(g));
+    }
+    return AMB_OK;
+}
+
+template <typename T>
+std::vector<BatchDesc<T>> make_descs(const amb_batch* b)
+{
+    std::vector<BatchDesc<T>> d(b->desc.size());
+    for (size_t r = 0; r < b->desc.size(); ++r) {
+        const amb_realisation& s = b->desc[r];
+        d[r].off = b->off[r];
+        d[r].ny = (int)s.ny;
+        d[r].nx = (int)s.nx;
+        d[r].nx4 = (s.nx % 4) == 0;
+        d[r].ntx = 0;  // set per launch group below
+        d[r].key0 = (uint32_t)s.seed;
+        d[r].key1 = (uint32_t)(s.seed >> 32);
+        d[r].k = make_coef<T>(s.stencil, s.model);
+    }
+    return d;
+}
+
+// Descriptors carry ntx of the realisation's own tile shape.
+int upload_descs(amb_batch* b)
+{
+    hipError_t e;
+    if (b->precision == AMB_F32) {
+        auto d = make_descs<float>(b);
+        for (size_t r = 0; r < d.size(); ++r) d[r].ntx = grid_tiles(d[r].nx, tile_shape(b->precision, b->tile_of[r]).tw);
+        e = hipMemcpy(b->d_desc, d.data(), d.size() * sizeof(d[0]), hipMemcpyHostToDevice);
+    } else {
+        auto d = make_descs<double>(b);
+        for (size_t r = 0; r < d.size(); ++r) d[r].ntx = grid_tiles(d[r].nx, tile_shape(b->precision, b->tile_of[r]).tw);
+        e = hipMemcpy(b->d_desc, d.data(), d.size() * sizeof(d[0]), hipMemcpyHostToDevice);
+    }
+    return e == hipSuccess ? AMB_OK : fail_hip(e, "descriptor upload");
+}
+
+hipError_t launch_groups(amb_batch* b, int parity_in, const long long* step, int step_off)
+{
+    for (const Group& g : b->groups) {
+        const hipError_t e = launch_batch(b->precision, g.tile, b->d_desc, g.d_tiles, g.ntiles, b->buf[parity_in & 1],
+                                          b->buf[(parity_in + 1) & 1], step, step_off, b->stream);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+int build_graph(amb_batch* b, int parity)
+{
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return fail_hip(e, "hipStreamBeginCapture");
+    for (int k = 0; k < kBatchGraphSteps && e == hipSuccess; ++k) e = launch_groups(b, parity + k, b->d_step, k);
+    if (e == hipSuccess) e = launch_advance(b->d_step, kBatchGraphSteps, b->stream);
+    const hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+    if (e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail_hip(e, "batched launch during graph capture");
+    }
+    if (e2 != hipSuccess) return fail_hip(e2, "hipStreamEndCapture");
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail_hip(e, "hipGraphInstantiate");
+    b->graph[parity] = exec;
+    return AMB_OK;
+}
+
+int enqueue(amb_batch* b, int64_t nsteps)
+{
+    hipError_t e;
+    int64_t left = nsteps;
+    if (left >= kBatchGraphSteps) {
+        const int p = b->cur;
+        if (!b->graph[p]) {
+            const int rc = build_graph(b, p);
+            if (rc != AMB_OK) return rc;
+        }
+        const int64_t reps = left / kBatchGraphSteps;
+        for (int64_t r = 0; r < reps; ++r)
+            if ((e = hipGraphLaunch(b->graph[p], b->stream)) != hipSuccess) return fail_hip(e, "hipGraphLaunch");
+        left -= reps * kBatchGraphSteps;
+    }
+    for (int k = 0; k < left; ++k)
+        if ((e = launch_groups(b, b->cur + k, b->d_step, k)) != hipSuccess) return fail_hip(e, "batched launch");
+    if (left > 0) {
+        if ((e = launch_advance(b->d_step, left, b->stream)) != hipSuccess) return fail_hip(e, "advance launch");
+        b->cur = (int)((b->cur + left) & 1);
+    }
+    return AMB_OK;
+}
+
+int reduce_range(amb_batch* b, int64_t off, int64_t n, double* sum, int64_t* bad)
+{
+    hipError_t e = b->precision == AMB_F32
+                       ? launch_reduce_f32(static_cast<const float*>(b->buf[b->cur]) + off, n, b->d_part,
+                                           b->d_bad_part, b->d_sum, b->d_bad, b->stream)
+                       : launch_reduce_f64(static_cast<const double*>(b->buf[b->cur]) + off, n, b->d_part,
+                                           b->d_bad_part, b->d_sum, b->d_bad, b->stream);
+    if (e != hipSuccess) return fail_hip(e, "reduction launch");
+    if ((e = hipMemcpyAsync(b->h_sum, b->d_sum, sizeof(double), hipMemcpyDeviceToHost, b->stream)) != hipSuccess)
+        return fail_hip(e, "hipMemcpyAsync");
+    if ((e = hipMemcpyAsync(b->h_bad, b->d_bad, sizeof(unsigned long long), hipMemcpyDeviceToHost, b->stream)) !=
+        hipSuccess)
+        return fail_hip(e, "hipMemcpyAsync");
+    if ((e = hipStreamSynchronize(b->stream)) != hipSuccess) return fail_hip(e, "hipStreamSynchronize");
+    *sum = *b->h_sum;
+    *bad = (int64_t)*b->h_bad;
+    return AMB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int amb_batch_create(const amb_realisation* desc, int64_t count, const double* const* fields, int precision,
+                     amb_batch** out)
+{
+    clear_error();
+    if (!out) return fail(AMB_E_ARG, "out must not be null");
+    *out = nullptr;
+    if (!desc || !fields || count <= 0) return fail(AMB_E_ARG, "need at least one realisation");
+    if (count > (1 << 24)) return fail(AMB_E_ARG, "too many realisations");
+    if (precision != AMB_F32 && precision != AMB_F64) return fail(AMB_E_ARG, "precision must be AMB_F32 or AMB_F64");
+    int64_t total = 0;
+    for (int64_t r = 0; r < count; ++r) {
+        const int rc = validate_params(desc[r].ny, desc[r].nx, &desc[r].stencil, &desc[r].model);
+        if (rc != AMB_OK) return rc;
+        if (!fields[r]) return fail(AMB_E_ARG, "field pointer must not be null");
+        total += desc[r].ny * desc[r].nx;
+    }
+    int rc = check_device();
+    if (rc != AMB_OK) return rc;
+    amb_batch* b = new (std::nothrow) amb_batch();
+    if (!b) return fail(AMB_E_ARG, "out of host memory");
+    b->precision = precision;
+    b->esize = precision == AMB_F32 ? 4 : 8;
+    b->desc.assign(desc, desc + count);
+    b->total = total;
+    b->off.resize(count);
+    int64_t o = 0;
+    for (int64_t r = 0; r < count; ++r) {
+        b->off[r] = o;
+        o += desc[r].ny * desc[r].nx;
+    }
+    b->tile_of.assign(count, 0);
+    hipError_t e;
+#define AMB_CHECK(call)                           \
+    do {                                          \
+        e = (call);                               \
+        if (e != hipSuccess) {                    \
+            const int code_ = fail_hip(e, #call); \
+            release(b);                           \
+            return code_;                         \
+        }                                         \
+    } while (0)
+    const size_t desc_bytes = (size_t)count * (precision == AMB_F32 ? sizeof(BatchDesc<float>) : sizeof(BatchDesc<double>));
+    AMB_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    AMB_CHECK(hipMalloc(&b->buf[0], (size_t)total * b->esize));
+    AMB_CHECK(hipMalloc(&b->buf[1], (size_t)total * b->esize));
+    AMB_CHECK(hipMalloc(&b->d_step, sizeof(long long)));
+    AMB_CHECK(hipMalloc(&b->d_step_scratch, sizeof(long long)));
+    AMB_CHECK(hipMalloc(&b->d_desc, desc_bytes));
+    AMB_CHECK(hipMalloc(&b->d_part, kReduceBlocks * sizeof(double)));
+    AMB_CHECK(hipMalloc(&b->d_bad_part, kReduceBlocks * sizeof(unsigned long long)));
+    AMB_CHECK(hipMalloc(&b->d_sum, sizeof(double)));
+    AMB_CHECK(hipMalloc(&b->d_bad, sizeof(unsigned long long)));
+    AMB_CHECK(hipHostMalloc(&b->h_sum, sizeof(double), hipHostMallocDefault));
+    AMB_CHECK(hipHostMalloc(&b->h_bad, sizeof(unsigned long long), hipHostMallocDefault));
+    AMB_CHECK(hipEventCreate(&b->ev0));
+    AMB_CHECK(hipEventCreate(&b->ev1));
+    const long long zero = 0;
+    AMB_CHECK(hipMemcpy(b->d_step, &zero, sizeof(long long), hipMemcpyHostToDevice));
+    // Upload fields (converted to the storage precision on the host).
+    for (int64_t r = 0; r < count; ++r) {
+        const size_t n = (size_t)(desc[r].ny * desc[r].nx);
+        if (precision == AMB_F32) {
+            std::vector<float> tmp(n);
+            for (size_t i = 0; i < n; ++i) tmp[i] = (float)fields[r][i];
+            AMB_CHECK(hipMemcpy(static_cast<float*>(b->buf[0]) + b->off[r], tmp.data(), n * sizeof(float),
+                                hipMemcpyHostToDevice));
+        } else {
+            AMB_CHECK(hipMemcpy(static_cast<double*>(b->buf[0]) + b->off[r], fields[r], n * sizeof(double),
+                                hipMemcpyHostToDevice));
+        }
+    }
+#undef AMB_CHECK
+    if ((rc = upload_descs(b)) != AMB_OK || (rc = rebuild_groups(b)) != AMB_OK) {
+        release(b);
+        return rc;
+    }
+    *out = b;
+    return AMB_OK;
+}
+
+int amb_batch_destroy(amb_batch* b)
+{
+    clear_error();
+    release(b);
+    return AMB_OK;
+}
+
+int amb_batch_run(amb_batch* b, int64_t nsteps)
+{
+    clear_error();
+    if (!b) return fail(AMB_E_ARG, "batch must not be null");
+    if (nsteps < 0) return fail(AMB_E_ARG, "nsteps must be non-negative");
+    int rc = enqueue(b, nsteps);
+    if (rc != AMB_OK) return rc;
+    double s;
+    int64_t bad;
+    if ((rc = reduce_range(b, 0, b->total, &s, &bad)) != AMB_OK) return rc;
+    if (bad != 0) return fail(AMB_E_NONFINITE, "an unknown numerical error occurred during simulation");
+    return AMB_OK;
+}
+
+int amb_batch_run_timed(amb_batch* b, int64_t nsteps, double* device_ms)
+{
+    clear_error();
+    if (!b || !device_ms) return fail(AMB_E_ARG, "batch and device_ms must not be null");
+    if (nsteps < 0) return fail(AMB_E_ARG, "nsteps must be non-negative");
+    hipError_t e;
+    if ((e = hipEventRecord(b->ev0, b->stream)) != hipSuccess) return fail_hip(e, "hipEventRecord");
+    int rc = enqueue(b, nsteps);
+    if (rc != AMB_OK) return rc;
+    if ((e = hipEventRecord(b->ev1, b->stream)) != hipSuccess) return fail_hip(e, "hipEventRecord");
+    if ((e = hipEventSynchronize(b->ev1)) != hipSuccess) return fail_hip(e, "hipEventSynchronize");
+    float ms = 0.f;
+    if ((e = hipEventElapsedTime(&ms, b->ev0, b->ev1)) != hipSuccess) return fail_hip(e, "hipEventElapsedTime");
+    *device_ms = ms;
+    double s;
+    int64_t bad;
+    if ((rc = reduce_range(b, 0, b->total, &s, &bad)) != AMB_OK) return rc;
+    if (bad != 0) return fail(AMB_E_NONFINITE, "an unknown numerical error occurred during simulation");
+    return AMB_OK;
+}
+
+int amb_batch_get_field(amb_batch* b, int64_t index, double* out)
+{
+    clear_error();
+    if (!b || !out) return fail(AMB_E_ARG, "batch and out must not be null");
+    if (index < 0 || index >= (int64_t)b->desc.size()) return fail(AMB_E_ARG, "realisation index out of range");
+    const size_t n = (size_t)(b->desc[index].ny * b->desc[index].nx);
+    hipError_t e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return fail_hip(e, "hipStreamSynchronize");
+    if (b->precision == AMB_F32) {
+        std::vector<float> tmp(n);
+        e = hipMemcpy(tmp.data(), static_cast<const float*>(b->buf[b->cur]) + b->off[index], n * sizeof(float),
+                      hipMemcpyDeviceToHost);
+        for (size_t i = 0; i < n; ++i) out[i] = (double)tmp[i];
+    } else {
+        e = hipMemcpy(out, static_cast<const double*>(b->buf[b->cur]) + b->off[index], n * sizeof(double),
+                      hipMemcpyDeviceToHost);
+    }
+    return e == hipSuccess ? AMB_OK : fail_hip(e, "hipMemcpy (field download)");
+}
+
+int amb_batch_mass(amb_batch* b, int64_t index, double* sum, int64_t* nonfinite)
+{
+    clear_error();
+    if (!b) return fail(AMB_E_ARG, "batch must not be null");
+    if (index < 0 || index >= (int64_t)b->desc.size()) return fail(AMB_E_ARG, "realisation index out of range");
+    double s;
+    int64_t bad;
+    const int rc = reduce_range(b, b->off[index], b->desc[index].ny * b->desc[index].nx, &s, &bad);
+    if (rc != AMB_OK) return rc;
+    if (sum) *sum = s;
+    if (nonfinite) *nonfinite = bad;
+    return AMB_OK;
+}
+
+int amb_batch_set_tiles(amb_batch* b, const int* tile_per_realisation)
+{
+    clear_error();
+    if (!b || !tile_per_realisation) return fail(AMB_E_ARG, "batch and tiles must not be null");
+    const int nt = tile_count(b->precision);
+    for (size_t r = 0; r < b->desc.size(); ++r)
+        if (tile_per_realisation[r] < 0 || tile_per_realisation[r] >= nt) return fail(AMB_E_ARG, "tile out of range");
+    hipError_t e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return fail_hip(e, "hipStreamSynchronize");
+    b->tile_of.assign(tile_per_realisation, tile_per_realisation + b->desc.size());
+    int rc = upload_descs(b);
+    if (rc == AMB_OK) rc = rebuild_groups(b);
+    return rc;
+}
+
+int amb_batch_get_tiles(amb_batch* b, int* tile_per_realisation)
+{
+    if (!b || !tile_per_realisation) return fail(AMB_E_ARG, "batch and tiles must not be null");
+    std::copy(b->tile_of.begin(), b->tile_of.end(), tile_per_realisation);
+    return AMB_OK;
+}
+
+// Autotuner: for every distinct (ny, nx) class, time `reps` steps of each compiled
+// tile shape on that class alone (on a saved copy of the state, with a scratch
+// step counter) and keep the fastest.  The field and step counter are restored.
+int amb_batch_autotune(amb_batch* b, int reps, int* chosen)
+{
+    clear_error();
+    if (!b) return fail(AMB_E_ARG, "batch must not be null");
+    if (reps < 1) reps = 4;
+    hipError_t e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return fail_hip(e, "hipStreamSynchronize");
+    std::map<std::pair<int64_t, int64_t>, std::vector<int>> classes;
+    for (size_t r = 0; r < b->desc.size(); ++r) classes[{b->desc[r].ny, b->desc[r].nx}].push_back((int)r);
+    void* saved = nullptr;
+    if ((e = hipMalloc(&saved, (size_t)b->total * b->esize)) != hipSuccess) return fail_hip(e, "hipMalloc");
+    e = hipMemcpy(saved, b->buf[b->cur], (size_t)b->total * b->esize, hipMemcpyDeviceToDevice);
+    const long long zero = 0;
+    if (e == hipSuccess) e = hipMemcpy(b->d_step_scratch, &zero, sizeof(long long), hipMemcpyHostToDevice);
+    const int nt = tile_count(b->precision);
+    std::vector<int> best_tile(b->desc.size(), 0);
+    for (auto& kv : classes) {
+        if (e != hipSuccess) break;
+        float best = 1e30f;
+        int best_t = 0;
+        for (int t = 0; t < nt && e == hipSuccess; ++t) {
+            // Descriptors must carry ntx for this tile shape while it is timed.
+            std::vector<int> trial = b->tile_of;
+            for (int r : kv.second) trial[r] = t;
+            std::swap(trial, b->tile_of);
+            int rc = upload_descs(b);
+            std::swap(trial, b->tile_of);
+            if (rc != AMB_OK) {
+                (void)hipFree(saved);
+                return rc;
+            }
+            const std::vector<int2> list = tile_list(b, kv.second, t);
+            int2* d_tiles = nullptr;
+            e = hipMalloc(&d_tiles, list.size() * sizeof(int2));
+            if (e == hipSuccess)
+                e = hipMemcpy(d_tiles, list.data(), list.size() * sizeof(int2), hipMemcpyHostToDevice);
+            // one warm-up step, then `reps` timed steps
+            for (int k = 0; k <= reps && e == hipSuccess; ++k) {
+                if (k == 1) e = hipEventRecord(b->ev0, b->stream);
+                if (e == hipSuccess)
+                    e = launch_batch(b->precision, t, b->d_desc, d_tiles, (int)list.size(), b->buf[(b->cur + k) & 1],
+                                     b->buf[(b->cur + k + 1) & 1], b->d_step_scratch, 0, b->stream);
+            }
+            if (e == hipSuccess) e = hipEventRecord(b->ev1, b->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(b->ev1);
+            float ms = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, b->ev0, b->ev1);
+            if (d_tiles) (void)hipFree(d_tiles);
+            if (e == hipSuccess && ms < best) {
+                best = ms;
+                best_t = t;
+            }
+        }
+        for (int r : kv.second) best_tile[r] = best_t;
+    }
+    if (e == hipSuccess) e = hipMemcpy(b->buf[b->cur], saved, (size_t)b->total * b->esize, hipMemcpyDeviceToDevice);
+    (void)hipFree(saved);
+    if (e != hipSuccess) return fail_hip(e, "autotune");
+    if (chosen) std::copy(best_tile.begin(), best_tile.end(), chosen);
+    return amb_batch_set_tiles(b, best_tile.data());
+}
+
+}  // extern "C"

@@ -1,0 +1,71 @@
+// amb_kernels.h -- host-side declarations of the HIP kernels' launchers (internal).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace amb {
+
+// Coefficients of one step with dx, dy, dt and the model folded in (host-computed
+// in double, cast to the arithmetic type T).
+template <typename T>
+struct StepCoef {
+    T a, b, c, kappa, lam, zeta;
+    T gxc, gxe, gyc, gye;  // G3: centre/edge smoothing weights times 1/dx (1/dy)
+    T lcc, lex, ley, lck;  // L3: centre, x-neighbours, y-neighbours, corners
+    T p0;                  // D8oD8 centre weight of -(mu_p'' along x and y)
+    T px[9], py[9];        // D8oD8 weights (index 1..8) times -1/dx^2, -1/dy^2
+    T lxx[3], lyy[3];      // G3oG3 smoothing weights times -1/dx^2, -1/dy^2
+    T zx, zy;              // 1/dx, 1/dy (divergence of the circulating current)
+    T nxw[5], nyw[5];      // D8 weights (index 1..4) times amp/dx, amp/dy (noise)
+    T dt;
+    T amp;                 // sqrt(2T) sqrt(1/(dt dx dy))
+};
+
+template <typename T>
+struct StepArgs {
+    const T* in;
+    T* out;
+    int ny, nx;
+    int nx4;          // nx % 4 == 0: column groups of 4 never straddle the periodic seam
+    int ntx, nty;     // tiles along x and y
+    int xcd_chunk;    // > 0: blockIdx remapped so each XCD gets a contiguous run of tiles
+    StepCoef<T> k;
+    uint32_t key0, key1;
+    const long long* step_base;  // device step counter; noise uses *step_base + step_off
+    int step_off;
+    const double* noise;         // explicit normals [2][ny][nx] of this step (noise mode 2)
+};
+
+// kNoiseRuntime: Philox when the coefficient amp is non-zero (batched realisations).
+enum NoiseMode { kNoiseNone = 0, kNoisePhilox = 1, kNoiseBuffer = 2, kNoiseRuntime = 3 };
+
+struct TileShape {
+    int tw, th;
+};
+
+// Tile shapes compiled per precision (index = autotuner choice).
+int tile_count(int precision);
+TileShape tile_shape(int precision, int index);
+
+// Launches one fused step.  Returns a hipError_t.
+hipError_t launch_step_f32(const StepArgs<float>& a, int tile, int noise, hipStream_t s);
+hipError_t launch_step_f64(const StepArgs<double>& a, int tile, int noise, hipStream_t s);
+hipError_t prepare_step_kernels();  // raises the dynamic-LDS limit of large tiles
+
+// Adds `n` to the device step counter (a one-lane kernel, graph-capturable).
+hipError_t launch_advance(long long* step, long long n, hipStream_t s);
+
+// Observables of the current step into double buffers (each n = ny*nx):
+// mup[n], mua[n], jp[2n], jl[2n], jz[2n], jn[2n].
+hipError_t launch_observables_f32(const StepArgs<float>& a, double* obs, hipStream_t s);
+hipError_t launch_observables_f64(const StepArgs<double>& a, double* obs, hipStream_t s);
+
+// FP64 sum and non-finite count of a field: deterministic two-pass reduction.
+constexpr int kReduceBlocks = 512;
+hipError_t launch_reduce_f32(const float* phi, int64_t n, double* part, unsigned long long* bad_part,
+                             double* sum, unsigned long long* bad, hipStream_t s);
+hipError_t launch_reduce_f64(const double* phi, int64_t n, double* part, unsigned long long* bad_part,
+                             double* sum, unsigned long long* bad, hipStream_t s);
+
+}  // namespace amb

@@ -1,0 +1,79 @@
+// amb_math.h -- stencil weights, Philox4x32-10 and Box-Muller shared by the HIP kernels.
+//
+// Discretisation of one explicit-Euler step of Active Model B+ (the reference's
+// scheme, restated from its mathematics; reference citations below):
+//
+//   G3x = rows (0.1, 0.3, 0.1) (x) columns (-1, 0, 1), scaled 1/dx      src/finite_difference.h:379-387
+//   G3y = transpose of G3x, scaled 1/dy                                    src/finite_difference.h:404-407
+//   L3  = S2x/dx^2 + S2y/dy^2, S2x = rows (-1/4, 3/2, -1/4) (x) (1,-2,1)    src/finite_difference.h:389-412,
+//                                                                          src/finite_difference.cuh:178-183
+//   D8  = 8th-order central first derivative, taps (4/5, -1/5, 4/105, -1/280) for offsets 1..4
+//                                                                          src/finite_difference.h:73
+//   mu_p = a phi + b phi^2 + c phi^3 - kappa L3 phi                        src/integrator.cu:34-37,99-100
+//   mu_a = lambda |G3 phi|^2                                               src/integrator.cu:101
+//   J_zeta = zeta (L3 phi) G3 phi,  J_noise = sqrt(2T/(dt dx dy)) xi       src/integrator.cu:103-109
+//   J_p = -D8 mu_p,  J_lambda = -G3 mu_a                                   src/integrator.cu:171-175
+//   div J = G3 . (J_lambda + J_zeta) + D8 . (J_p + J_noise);  phi -= dt div J
+//                                                                          src/integrator.cu:259-261
+//
+// The fused kernel never materialises J_p or J_lambda: D8(D8 mu_p) is applied as
+// one 17-tap stencil (the exact convolution of D8 with itself) and G3x(G3x mu_a)
+// as the separable 5x5 product rows (.01,.06,.11,.06,.01) (x) columns (1,0,-2,0,1).
+#pragma once
+
+#include <stdint.h>
+
+namespace amb {
+
+// D8 taps for offsets 1..4 (antisymmetric).
+constexpr double kD8[5] = {0.0, 4.0 / 5.0, -1.0 / 5.0, 4.0 / 105.0, -1.0 / 280.0};
+
+// D8 o D8 for offsets 0..8 (symmetric): sum_{k+l=m} d_k d_l, exact rationals.
+constexpr double kD8D8[9] = {
+    -480841.0 / 352800.0, 411.0 / 1225.0, 1213.0 / 2100.0, -11.0 / 35.0, 53.0 / 525.0,
+    -11.0 / 525.0,        127.0 / 44100.0, -1.0 / 3675.0,  1.0 / 78400.0};
+
+// Smoothing profile of G3 across its derivative axis, and of G3 o G3.
+constexpr double kG3S[2] = {0.3, 0.1};                 // centre, +-1
+constexpr double kG3G3S[3] = {0.11, 0.06, 0.01};       // centre, +-1, +-2
+
+// Noise mapping (identical in the CPU oracle): one Philox4x32-10 call per
+// (column group of 4, row, step, component); the 4 words give 4 normals by two
+// Box-Muller transforms.
+constexpr uint32_t kPhiloxM0 = 0xD2511F53u, kPhiloxM1 = 0xCD9E8D57u;
+constexpr uint32_t kPhiloxW0 = 0x9E3779B9u, kPhiloxW1 = 0xBB67AE85u;
+
+struct U4 {
+    uint32_t x, y, z, w;
+};
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
+__host__ __device__ inline U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)kPhiloxM0 * c.x;
+        const uint64_t p1 = (uint64_t)kPhiloxM1 * c.z;
+        U4 n;
+        n.x = (uint32_t)(p1 >> 32) ^ c.y ^ k0;
+        n.y = (uint32_t)p1;
+        n.z = (uint32_t)(p0 >> 32) ^ c.w ^ k1;
+        n.w = (uint32_t)p0;
+        c = n;
+        k0 += kPhiloxW0;
+        k1 += kPhiloxW1;
+    }
+    return c;
+}
+
+__host__ __device__ inline U4 noise_counter(uint32_t group, uint32_t row, uint64_t step, int comp)
+{
+    U4 c;
+    c.x = group;
+    c.y = row;
+    c.z = (uint32_t)step;
+    c.w = ((uint32_t)(step >> 32) & 0x7fffffffu) | ((uint32_t)comp << 31);
+    return c;
+}
+
+}  // namespace amb

@@ -1,0 +1,318 @@
+// amb_step.hip -- fused explicit-Euler step of Active Model B+ for gfx950 (CDNA4).
+//
+// One launch per step reads phi^n and writes phi^{n+1}; nothing else touches HBM
+// (8 B per site in FP32, 16 B in FP64).  The reference needs three launches per
+// step and ~280 B per site (src/integrator.cu:46-262, host loop :507-518).
+//
+// Per tile of TW x TH outputs (one 256-thread workgroup, four wave64s):
+//   stage 0  phi on rows [-9, TH+9) x cols [-9, TW+9) -> LDS (periodic wrap)
+//   stage A  on the cross-shaped halo region: G3x phi, G3y phi, L3 phi, then
+//            mu_p (rows/cols +-8), mu_a = lambda|G3 phi|^2 (box +-2) and the
+//            circulating current zeta L3phi G3phi (box +-1) -> LDS
+//   stage N  conserved-noise normals xi_x (cols +-4), xi_y (rows +-4) from
+//            counter-based Philox4x32-10 -> LDS (aliases the phi buffer)
+//   stage B  div J per output from LDS: D8oD8 on mu_p (17 taps per axis),
+//            G3oG3 on mu_a (5x5 separable), G3 on the circulating current, D8 on
+//            the noise; phi^{n+1} = phi^n - dt div J.
+// A site's noise depends only on (seed, step, row, column, component), so the
+// halo copies that neighbouring tiles generate are bit-identical to the owner's.
+#include <hip/hip_runtime.h>
+
+#include "amb_kernels.h"
+#include "amb_math.h"
+#include "amb_tile.h"
+
+namespace amb {
+
+// ---- single-field kernel ----------------------------------------------------------
+
+template <typename T, int TW, int TH, int NOISE>
+__global__ __launch_bounds__(kThreads) void step_kernel(const StepArgs<T> p)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    // Tile index; with xcd_chunk the round-robin XCD dispatch gets contiguous tiles.
+    int t = blockIdx.x;
+    if (p.xcd_chunk > 0) t = (t & 7) * p.xcd_chunk + (t >> 3);
+    const int ty = t / p.ntx, tx = t - ty * p.ntx;
+    const uint64_t step = NOISE == kNoisePhilox ? (uint64_t)(*p.step_base) + (uint64_t)p.step_off : 0;
+    step_tile<T, TW, TH, NOISE>(reinterpret_cast<T*>(smem_raw), p.in, p.out, p.ny, p.nx, p.nx4, tx * TW, ty * TH, p.k,
+                                p.key0, p.key1, step, p.noise);
+}
+
+// ---- tile table ---------------------------------------------------------------
+
+// FP32 and FP64 tile shapes (autotuner search space).
+static const TileShape kTilesF32[] = {{64, 32}, {64, 16}, {32, 32}, {64, 64}};
+static const TileShape kTilesF64[] = {{32, 32}, {64, 16}, {32, 16}, {64, 32}};
+
+int tile_count(int precision)
+{
+    return precision == 32 ? (int)(sizeof(kTilesF32) / sizeof(kTilesF32[0]))
+                           : (int)(sizeof(kTilesF64) / sizeof(kTilesF64[0]));
+}
+
+TileShape tile_shape(int precision, int index)
+{
+    return precision == 32 ? kTilesF32[index] : kTilesF64[index];
+}
+
+template <typename T, int TW, int TH, int NOISE>
+static hipError_t launch_one(const StepArgs<T>& a, hipStream_t s)
+{
+    using L = Lds<T, TW, TH>;
+    const int nblocks = a.ntx * a.nty;
+    hipLaunchKernelGGL((step_kernel<T, TW, TH, NOISE>), dim3(nblocks), dim3(kThreads), L::bytes, s, a);
+    return hipGetLastError();
+}
+
+template <typename T, int TW, int TH>
+static hipError_t launch_noise(const StepArgs<T>& a, int noise, hipStream_t s)
+{
+    switch (noise) {
+    case kNoiseNone: return launch_one<T, TW, TH, kNoiseNone>(a, s);
+    case kNoisePhilox: return launch_one<T, TW, TH, kNoisePhilox>(a, s);
+    default: return launch_one<T, TW, TH, kNoiseBuffer>(a, s);
+    }
+}
+
+hipError_t launch_step_f32(const StepArgs<float>& a, int tile, int noise, hipStream_t s)
+{
+    switch (tile) {
+    case 0: return launch_noise<float, 64, 32>(a, noise, s);
+    case 1: return launch_noise<float, 64, 16>(a, noise, s);
+    case 2: return launch_noise<float, 32, 32>(a, noise, s);
+    default: return launch_noise<float, 64, 64>(a, noise, s);
+    }
+}
+
+hipError_t launch_step_f64(const StepArgs<double>& a, int tile, int noise, hipStream_t s)
+{
+    switch (tile) {
+    case 0: return launch_noise<double, 32, 32>(a, noise, s);
+    case 1: return launch_noise<double, 64, 16>(a, noise, s);
+    case 2: return launch_noise<double, 32, 16>(a, noise, s);
+    default: return launch_noise<double, 64, 32>(a, noise, s);
+    }
+}
+
+template <typename T, int TW, int TH>
+static hipError_t raise_lds_limit()
+{
+    using L = Lds<T, TW, TH>;
+    hipError_t e = hipSuccess;
+    if (L::bytes > 65536) {
+        e = hipFuncSetAttribute((const void*)step_kernel<T, TW, TH, kNoiseNone>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)L::bytes);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void*)step_kernel<T, TW, TH, kNoisePhilox>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)L::bytes);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void*)step_kernel<T, TW, TH, kNoiseBuffer>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)L::bytes);
+    }
+    return e;
+}
+
+hipError_t prepare_step_kernels()
+{
+    hipError_t e;
+    if ((e = raise_lds_limit<float, 64, 32>()) != hipSuccess) return e;
+    if ((e = raise_lds_limit<float, 64, 16>()) != hipSuccess) return e;
+    if ((e = raise_lds_limit<float, 32, 32>()) != hipSuccess) return e;
+    if ((e = raise_lds_limit<float, 64, 64>()) != hipSuccess) return e;
+    if ((e = raise_lds_limit<double, 32, 32>()) != hipSuccess) return e;
+    if ((e = raise_lds_limit<double, 64, 16>()) != hipSuccess) return e;
+    if ((e = raise_lds_limit<double, 32, 16>()) != hipSuccess) return e;
+    return raise_lds_limit<double, 64, 32>();
+}
+
+// ---- step counter -------------------------------------------------------------
+
+__global__ void advance_kernel(long long* step, long long n)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) step[0] += n;
+}
+
+hipError_t launch_advance(long long* step, long long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(64), 0, s, step, n);
+    return hipGetLastError();
+}
+
+// ---- observables (getters) ------------------------------------------------------
+// A plain per-site formulation with wrapped global loads: a second, independent
+// device implementation of the same terms (two passes, like the reference's
+// calculate_forces / calculate_local_currents, src/integrator.cu:46-176).
+
+template <typename T>
+__device__ __forceinline__ T ld(const T* f, int ny, int nx, int r, int c)
+{
+    return f[(size_t)wrap_index(r, ny) * nx + wrap_index(c, nx)];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void obs_local_kernel(const StepArgs<T> p, double* obs)
+{
+    const int ny = p.ny, nx = p.nx;
+    const size_t n = (size_t)ny * nx;
+    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const int r = (int)(i / nx), c = (int)(i - (size_t)r * nx);
+    const StepCoef<T>& K = p.k;
+    const T* f = p.in;
+    const T pmm = ld(f, ny, nx, r - 1, c - 1), pm0 = ld(f, ny, nx, r - 1, c), pmp = ld(f, ny, nx, r - 1, c + 1);
+    const T p0m = ld(f, ny, nx, r, c - 1), p00 = ld(f, ny, nx, r, c), p0p = ld(f, ny, nx, r, c + 1);
+    const T ppm = ld(f, ny, nx, r + 1, c - 1), pp0 = ld(f, ny, nx, r + 1, c), ppp = ld(f, ny, nx, r + 1, c + 1);
+    const T gx = K.gxe * ((pmp - pmm) + (ppp - ppm)) + K.gxc * (p0p - p0m);
+    const T gy = K.gye * ((ppm - pmm) + (ppp - pmp)) + K.gyc * (pp0 - pm0);
+    const T lap = K.lcc * p00 + K.lex * (p0m + p0p) + K.ley * (pm0 + pp0) + K.lck * ((pmm + pmp) + (ppm + ppp));
+    double* mup = obs;
+    double* mua = obs + n;
+    double* jz = obs + 6 * n;
+    double* jn = obs + 8 * n;
+    mup[i] = (double)(p00 * (K.a + p00 * (K.b + p00 * K.c)) - K.kappa * lap);
+    mua[i] = (double)(K.lam * (gy * gy + gx * gx));
+    const T zl = K.zeta * lap;
+    jz[i] = (double)(zl * gy);
+    jz[n + i] = (double)(zl * gx);
+    T xy = 0, xx = 0;
+    if (p.noise) {
+        xy = (T)p.noise[i];
+        xx = (T)p.noise[n + i];
+    } else if (K.amp != T(0)) {
+        const uint64_t step = (uint64_t)(*p.step_base);
+        xy = normal1<T>(r, c, step, 0, p.key0, p.key1);
+        xx = normal1<T>(r, c, step, 1, p.key0, p.key1);
+    }
+    jn[i] = (double)(K.amp * xy);
+    jn[n + i] = (double)(K.amp * xx);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void obs_current_kernel(const StepArgs<T> p, double* obs)
+{
+    const int ny = p.ny, nx = p.nx;
+    const size_t n = (size_t)ny * nx;
+    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const int r = (int)(i / nx), c = (int)(i - (size_t)r * nx);
+    const StepCoef<T>& K = p.k;
+    const double* mup = obs;
+    const double* mua = obs + n;
+    double* jp = obs + 2 * n;
+    double* jl = obs + 4 * n;
+    // J_p = -D8 mu_p (src/integrator.cu:171-172); K.zx = 1/dx, K.zy = 1/dy.
+    T dy8 = 0, dx8 = 0;
+#pragma unroll
+    for (int k = 1; k <= 4; ++k) {
+        dy8 += (T)kD8[k] * ((T)ld(mup, ny, nx, r + k, c) - (T)ld(mup, ny, nx, r - k, c));
+        dx8 += (T)kD8[k] * ((T)ld(mup, ny, nx, r, c + k) - (T)ld(mup, ny, nx, r, c - k));
+    }
+    jp[i] = (double)(-K.zy * dy8);
+    jp[n + i] = (double)(-K.zx * dx8);
+    // J_lambda = -G3 mu_a (src/integrator.cu:174-175)
+    const T amm = (T)ld(mua, ny, nx, r - 1, c - 1), am0 = (T)ld(mua, ny, nx, r - 1, c), amp_ = (T)ld(mua, ny, nx, r - 1, c + 1);
+    const T a0m = (T)ld(mua, ny, nx, r, c - 1), a0p = (T)ld(mua, ny, nx, r, c + 1);
+    const T apm = (T)ld(mua, ny, nx, r + 1, c - 1), ap0 = (T)ld(mua, ny, nx, r + 1, c), app = (T)ld(mua, ny, nx, r + 1, c + 1);
+    const T gx = K.gxe * ((amp_ - amm) + (app - apm)) + K.gxc * (a0p - a0m);
+    const T gy = K.gye * ((apm - amm) + (app - amp_)) + K.gyc * (ap0 - am0);
+    jl[i] = (double)(-gy);
+    jl[n + i] = (double)(-gx);
+}
+
+template <typename T>
+static hipError_t launch_obs(const StepArgs<T>& a, double* obs, hipStream_t s)
+{
+    const size_t n = (size_t)a.ny * a.nx;
+    const unsigned blocks = (unsigned)((n + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(obs_local_kernel<T>, dim3(blocks), dim3(kThreads), 0, s, a, obs);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(obs_current_kernel<T>, dim3(blocks), dim3(kThreads), 0, s, a, obs);
+    return hipGetLastError();
+}
+
+hipError_t launch_observables_f32(const StepArgs<float>& a, double* obs, hipStream_t s) { return launch_obs(a, obs, s); }
+hipError_t launch_observables_f64(const StepArgs<double>& a, double* obs, hipStream_t s) { return launch_obs(a, obs, s); }
+
+// ---- FP64 mass and finite check ----------------------------------------------------
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void reduce_partial_kernel(const T* phi, int64_t n, double* part,
+                                                                  unsigned long long* bad_part)
+{
+    __shared__ double ssum[kThreads];
+    __shared__ unsigned long long sbad[kThreads];
+    double s = 0.0;
+    unsigned long long b = 0;
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        const double v = (double)phi[i];
+        if (isfinite(v)) s += v; else ++b;
+    }
+    ssum[threadIdx.x] = s;
+    sbad[threadIdx.x] = b;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            ssum[threadIdx.x] += ssum[threadIdx.x + w];
+            sbad[threadIdx.x] += sbad[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = ssum[0];
+        bad_part[blockIdx.x] = sbad[0];
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void reduce_final_kernel(const double* part, const unsigned long long* bad_part,
+                                                                int nparts, double* sum, unsigned long long* bad)
+{
+    __shared__ double ssum[kThreads];
+    __shared__ unsigned long long sbad[kThreads];
+    double s = 0.0;
+    unsigned long long b = 0;
+    for (int i = threadIdx.x; i < nparts; i += kThreads) {
+        s += part[i];
+        b += bad_part[i];
+    }
+    ssum[threadIdx.x] = s;
+    sbad[threadIdx.x] = b;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            ssum[threadIdx.x] += ssum[threadIdx.x + w];
+            sbad[threadIdx.x] += sbad[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        sum[0] = ssum[0];
+        bad[0] = sbad[0];
+    }
+}
+
+template <typename T>
+static hipError_t launch_reduce(const T* phi, int64_t n, double* part, unsigned long long* bad_part, double* sum,
+                                unsigned long long* bad, hipStream_t s)
+{
+    hipLaunchKernelGGL(reduce_partial_kernel<T>, dim3(kReduceBlocks), dim3(kThreads), 0, s, phi, n, part, bad_part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(reduce_final_kernel, dim3(1), dim3(kThreads), 0, s, part, bad_part, kReduceBlocks, sum, bad);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_f32(const float* phi, int64_t n, double* part, unsigned long long* bad_part, double* sum,
+                             unsigned long long* bad, hipStream_t s)
+{
+    return launch_reduce(phi, n, part, bad_part, sum, bad, s);
+}
+
+hipError_t launch_reduce_f64(const double* phi, int64_t n, double* part, unsigned long long* bad_part, double* sum,
+                             unsigned long long* bad, hipStream_t s)
+{
+    return launch_reduce(phi, n, part, bad_part, sum, bad, s);
+}
+
+}  // namespace amb

@@ -1,0 +1,206 @@
+#!/usr/bin/env python3
+"""Benchmark of the fused Active Model B+ step on one MI355X.
+
+Default workload: a single 4096 x 4096 periodic field, FP32 state, the roofline
+configuration (a=-0.25, b=0, c=0.25, kappa=1, lambda=0.5, zeta=-1, T=0.05,
+dt=5e-3, dx=dy=1, phi = 0.05 N(0,1) from numpy seed 3).  A "step" is one explicit
+Euler step of the whole field; the field stays resident in HBM.
+
+Prints ONE JSON line.  ``value`` is site-updates per second over the timed
+region (wall clock around exactly --steps steps, after --warmup untimed steps).
+``roofline`` prices the step kernel against HBM: algorithmic bytes per launch
+(read phi^n + write phi^{n+1} = 8 B/site FP32, 16 B/site FP64) over the average
+launch duration from HIP events on the integrator's own stream.  ``cpu_baseline``
+times the CPU FP64 oracle (oracle/, the checker) on this host on a bounded sample.
+
+Other workloads (--workload): c1 (256^2 FP64 spinodal), c2 (1024^2 FP32 noisy
+active), c3 (default), c3f64 (4096^2 FP64), c4 (batched ensemble of 512
+realisations, sides 64..512, vs sequential plain runs on the <=256^2 subset).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "active-model-bplus_amd"))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "site-updates/s, % HBM peak; batched vs plain; surrogate speedup at ≤2% S(k) err"
+
+WORKLOADS = {
+    "c1": dict(n=256, precision=64, seed=1, stencil=(1e-3, 1, 1), model=(-0.25, 0, 0.25, 1, 0, 0, 0),
+               init="spinodal", label="C1 passive spinodal 256x256 FP64"),
+    "c2": dict(n=1024, precision=32, seed=2, stencil=(5e-3, 1, 1), model=(-0.25, 0, 0.25, 1, -1, -4, 0.1),
+               init=("normal", -0.4, 0.01), label="C2 AMB+ noisy 1024x1024 FP32"),
+    "c3": dict(n=4096, precision=32, seed=3, stencil=(5e-3, 1, 1), model=(-0.25, 0, 0.25, 1, 0.5, -1, 0.05),
+               init=("normal", 0.0, 0.05), label="C3 AMB+ noisy 4096x4096 FP32 (roofline)"),
+    "c3f64": dict(n=4096, precision=64, seed=3, stencil=(5e-3, 1, 1), model=(-0.25, 0, 0.25, 1, 0.5, -1, 0.05),
+                  init=("normal", 0.0, 0.05), label="C3 AMB+ noisy 4096x4096 FP64"),
+}
+
+
+def initial_field(w):
+    rng = np.random.default_rng(w["seed"])
+    n = w["n"]
+    if w["init"] == "spinodal":
+        phi = 0.1 * rng.random((n, n))
+        return phi - phi.mean()
+    _, mean, std = w["init"]
+    return mean + std * rng.normal(size=(n, n))
+
+
+def cpu_baseline(w, phi, budget_s=15.0):
+    """Times the FP64 CPU oracle (OpenMP over rows) on this host."""
+    from oracle import oracle as o
+
+    o.lib()
+    threads = max(1, min(16, os.cpu_count() or 1, o.max_threads()))
+    n = phi.shape[0]
+    # bounded sample: a sub-grid sized so one step takes a fraction of the budget
+    side = n
+    while side > 256 and (side * side) > 4e6:
+        side //= 2
+    sub = np.ascontiguousarray(phi[:side, :side])
+    t0 = time.perf_counter()
+    o.run(sub, w["stencil"], w["model"], 1, seed=w["seed"], nthreads=threads)
+    one = time.perf_counter() - t0
+    steps = max(1, int(budget_s / max(one, 1e-6)))
+    t0 = time.perf_counter()
+    o.run(sub, w["stencil"], w["model"], steps, seed=w["seed"], nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": side * side * steps / dt, "unit": "site-updates/s", "cores": threads, "kind": "port",
+            "sample": f"oracle FP64, {side}x{side} periodic sub-grid of the same workload, {steps} steps, "
+                      f"{threads} OpenMP threads, {dt:.1f} s"}
+
+
+def read_traffic(name):
+    """HBM bytes per launch measured by rocprofv3 --pmc (committed under profiles/), or None."""
+    path = os.path.join(REPO, "profiles", f"{name}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def bench_single(args, key):
+    from activemodelbplus.integrator import Integrator, Model, Stencil
+
+    w = WORKLOADS[key]
+    phi = initial_field(w)
+    sim = Integrator(phi, Stencil(*w["stencil"]), Model(*w["model"]), seed=w["seed"], precision=w["precision"])
+    tile = sim.autotune(4) if args.autotune else 0
+    if args.warmup:
+        sim.run(args.warmup)
+    t0 = time.perf_counter()
+    device_ms = sim.run_timed(args.steps)
+    wall = time.perf_counter() - t0
+    sites = phi.size
+    value = sites * args.steps / wall
+    esize = 4 if w["precision"] == 32 else 8
+    bytes_per_launch = 2 * esize * sites
+    launch_s = device_ms / 1e3 / args.steps
+    achieved = bytes_per_launch / launch_s / 1e9
+    mass, bad = sim.mass()
+    out = {
+        "metric": METRIC, "value": value, "unit": "site-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if w["precision"] == 32 else "f64",
+        "data": "synthetic (seeded numpy normal initial field)",
+        "config": {"workload": w["label"], "grid": [w["n"], w["n"]], "stencil": w["stencil"],
+                   "model": dict(zip(["a", "b", "c", "kappa", "lamb", "zeta", "T"], w["model"])),
+                   "tile": list(sim_tile_shape(w["precision"], tile))},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(f"step_{key}")},
+        "device_ms_per_step": device_ms / args.steps,
+        "mass_fp64": mass, "nonfinite": bad,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w, phi)
+    return out
+
+
+def sim_tile_shape(precision, index):
+    from activemodelbplus.integrator import tile_shapes
+    return tile_shapes(precision)[index]
+
+
+def bench_batched(args):
+    """C4: 512 realisations, side in {64,128,256,512}, lambda~U[-2,2], zeta~U[-5,0],
+    phi0~U[-0.6,0.6], T in {0,0.05,0.1}; batched vs sequential plain on <=256^2."""
+    from activemodelbplus.integrator import Ensemble, Integrator, Model, Stencil
+
+    rng = np.random.default_rng(4)
+    count = args.realisations
+    sides = rng.choice([64, 128, 256, 512], size=count)
+    lam = rng.uniform(-2, 2, size=count)
+    zeta = rng.uniform(-5, 0, size=count)
+    phi0 = rng.uniform(-0.6, 0.6, size=count)
+    temp = rng.choice([0.0, 0.05, 0.1], size=count)
+    fields = [phi0[i] + 0.01 * rng.normal(size=(sides[i], sides[i])) for i in range(count)]
+    models = [Model(-0.25, 0, 0.25, 1, lam[i], zeta[i], temp[i]) for i in range(count)]
+    seeds = [(4 << 32) + i for i in range(count)]
+    st = Stencil(5e-3, 1, 1)
+    small = [i for i in range(count) if sides[i] <= 256]
+
+    def batched(idx, steps):
+        ens = Ensemble([fields[i] for i in idx], st, [models[i] for i in idx], [seeds[i] for i in idx], precision=32)
+        ens.autotune(2)
+        ens.run(args.warmup)
+        t0 = time.perf_counter()
+        ens.run(steps)
+        return time.perf_counter() - t0, ens
+
+    def sequential(idx, steps):
+        total = 0.0
+        for i in idx:
+            sim = Integrator(fields[i], st, models[i], seed=seeds[i], precision=32)
+            sim.run(min(args.warmup, 32))
+            t0 = time.perf_counter()
+            sim.run(steps)
+            total += time.perf_counter() - t0
+        return total
+
+    steps = args.steps
+    t_all, _ = batched(list(range(count)), steps)
+    t_small, _ = batched(small, steps)
+    seq_steps = max(32, steps // 10)
+    t_seq_small = sequential(small, seq_steps) * steps / seq_steps
+    sites_all = int(sum(s * s for s in sides))
+    sites_small = int(sum(sides[i] ** 2 for i in small))
+    value = sites_all * steps / t_all
+    return {
+        "metric": METRIC, "value": value, "unit": "site-updates/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": t_all * 1e3 / steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded numpy)",
+        "config": {"workload": f"C4 batched ensemble of {count} realisations", "sites": sites_all},
+        "batched_vs_plain_le256": t_seq_small / t_small,
+        "plain_sequential_le256_s_extrapolated_from": seq_steps,
+        "batched_le256_site_updates_per_s": sites_small * steps / t_small,
+    }
+
+
+def main():
+    p = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=2000)
+    p.add_argument("--warmup", type=int, default=64)
+    p.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c4"])
+    p.add_argument("--realisations", type=int, default=512)
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    p.add_argument("--autotune", action="store_true", help="pick the tile shape by timing")
+    args = p.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU slab "
+              "decomposition is a design, not a measured path.", file=sys.stderr)
+        sys.exit(2)
+    out = bench_batched(args) if args.workload == "c4" else bench_single(args, args.workload)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,171 @@
+/*
+ * amb.h -- C ABI of the MI355X Active Model B+ integrator (libamb.so).
+ *
+ * Plain C types only: doubles, 64-bit sizes and opaque handles.  Every call
+ * returns AMB_OK (0) or an error code; amb_last_error() gives a thread-local
+ * message for the last failure on the calling thread.  No C++ exception crosses
+ * this boundary.  A handle must not be used from two threads at once; separate
+ * handles are independent (all parameters travel as kernel arguments, there is
+ * no process-global device state).
+ *
+ * Field layout everywhere: row-major (ny, nx) float64, linear index col + row*nx,
+ * periodic in both axes.  Vector quantities are 2*ny*nx doubles laid out
+ * [component y | component x] (component 0 = rows, the reference's convention).
+ */
+#ifndef AMB_H
+#define AMB_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes -------------------------------------------------------- */
+#define AMB_OK 0
+#define AMB_E_ARG 1        /* invalid argument (shape, null pointer, parameter) */
+#define AMB_E_HIP 2        /* HIP runtime error (message carries hipGetErrorString) */
+#define AMB_E_NONFINITE 3  /* field became NaN/Inf during run (reference: CudaError) */
+#define AMB_E_NODEVICE 4   /* no HIP device visible */
+#define AMB_E_STATE 5      /* call not valid in the handle's current state */
+
+/* ---- parameter structs --------------------------------------------------- */
+
+/* Model parameters a, b, c, kappa, lambda, zeta, temperature.
+ * Replaces ActiveModelBPlusParams (src/parameters.h:53-77). */
+typedef struct {
+    double a, b, c, kappa, lamb, zeta, T;
+} amb_model;
+
+/* Time step and grid spacings.  Replaces HostStencilParams (src/parameters.h:23-41). */
+typedef struct {
+    double dt, dx, dy;
+} amb_stencil;
+
+#define AMB_F64 64
+#define AMB_F32 32
+
+/* Options with no reference counterpart (the reference is FP64 only and seeds its
+ * noise from std::random_device, src/integrator.cu:320-323).  Passing NULL means
+ * {precision = AMB_F64, seed = 0, timestep = 0, tile = 0}. */
+typedef struct {
+    int precision;     /* AMB_F64 or AMB_F32: storage and arithmetic of the field */
+    uint64_t seed;     /* Philox key of the conserved noise */
+    int64_t timestep;  /* initial step counter (exact resume: field, timestep, seed) */
+    int tile;          /* 0 = default tile; k > 0 selects tile index k-1 (see amb_tile_count) */
+} amb_options;
+
+typedef struct amb_handle amb_handle;
+
+/* ---- single field: the drop-in for class Integrator ---------------------- */
+
+/* Copies `field` (ny*nx doubles) to the device.
+ * Replaces Integrator::Integrator(const HostFieldRef&, Stencil, Model)
+ * (src/integrator.h:34, src/integrator.cu:288-333). */
+int amb_create(const double* field, int64_t ny, int64_t nx, const amb_stencil* stencil,
+               const amb_model* model, const amb_options* options, amb_handle** out);
+
+/* Frees every device resource.  Replaces Integrator::~Integrator (src/integrator.cu:353-366).
+ * NULL is accepted. */
+int amb_destroy(amb_handle* h);
+
+/* Advances nsteps explicit-Euler steps (nsteps >= 0), then checks the field for
+ * NaN/Inf and returns AMB_E_NONFINITE if any site is not finite.
+ * Replaces Integrator::run(int) (src/integrator.cu:497-538). */
+int amb_run(amb_handle* h, int64_t nsteps);
+
+/* Copies the current field (ny*nx doubles) to `out`.
+ * Replaces Integrator::get_field (src/integrator.cu:378-383). */
+int amb_get_field(amb_handle* h, double* out);
+
+/* Observables at the current step, i.e. the quantities the next step will use
+ * (noise included), as the reference's getters define them
+ * (src/integrator.cu:385-465, lazily computed at src/integrator.cu:477-495).
+ * Scalars fill ny*nx doubles, currents 2*ny*nx doubles. */
+#define AMB_OBS_MU 0                 /* get_chemical_potential (scalar) */
+#define AMB_OBS_MU_PASSIVE 1         /* get_passive_chemical_potential (scalar) */
+#define AMB_OBS_MU_ACTIVE 2          /* get_active_chemical_potential (scalar) */
+#define AMB_OBS_J 3                  /* get_current */
+#define AMB_OBS_J_DETERMINISTIC 4    /* get_deterministic_current */
+#define AMB_OBS_J_PASSIVE 5          /* get_passive_current */
+#define AMB_OBS_J_ACTIVE 6           /* get_active_current */
+#define AMB_OBS_J_LAMBDA 7           /* get_lambda_current */
+#define AMB_OBS_J_CIRCULATING 8      /* get_circulating_current */
+#define AMB_OBS_J_RANDOM 9           /* get_random_current */
+int amb_get_observable(amb_handle* h, int which, double* out);
+
+/* Step counter and parameters.  Replace Integrator::get_timestep, get_stencil,
+ * get_model (src/integrator.h:38-39,52-55). */
+int amb_get_timestep(const amb_handle* h, int64_t* timestep);
+int amb_get_params(const amb_handle* h, amb_stencil* stencil, amb_model* model);
+int amb_get_shape(const amb_handle* h, int64_t* ny, int64_t* nx);
+int amb_get_options(const amb_handle* h, amb_options* options);
+
+/* Sum of the field accumulated in FP64 and the number of non-finite sites.
+ * Replaces kernel::check_finite (src/integrator.cu:265-272) and adds the mass. */
+int amb_mass(amb_handle* h, double* sum, int64_t* nonfinite);
+
+/* Test-only: explicit unscaled normals for the next `nsteps` steps
+ * (nsteps * 2 * ny * nx doubles, step-major, each step [xi_y | xi_x]) replace the
+ * Philox stream until they are used up; the caller's buffer is copied.  Lets a
+ * CPU scheme and the GPU read one shared noise buffer.  No reference counterpart. */
+int amb_set_noise(amb_handle* h, const double* noise, int64_t nsteps);
+
+/* Like amb_run, and returns the device time of the steps, measured with HIP
+ * events recorded on the handle's own stream (the stream the step kernels are
+ * launched on).  No reference counterpart; used by the benchmark. */
+int amb_run_timed(amb_handle* h, int64_t nsteps, double* device_ms);
+
+/* Tile shapes compiled for the step kernel of a precision (autotuner search space). */
+int amb_tile_count(int precision, int* count);
+int amb_tile_shape(int precision, int index, int* tile_cols, int* tile_rows);
+int amb_set_tile(amb_handle* h, int index);
+
+/* Autotuner for one field: times `reps` steps of every compiled tile shape on a
+ * copy of the current state (field and step counter untouched), selects the
+ * fastest for this handle and returns its index in `chosen` (nullable). */
+int amb_autotune(amb_handle* h, int reps, int* chosen);
+
+/* Thread-local message of the last error on this thread ("" if none). */
+const char* amb_last_error(void);
+
+/* Number of visible HIP devices (0 without a GPU; never fails). */
+int amb_device_count(int* count);
+
+/* ---- batched ensemble (no reference counterpart) ------------------------- */
+
+/* One independent realisation: its own grid, stencil, model and noise seed. */
+typedef struct {
+    int64_t ny, nx;
+    amb_stencil stencil;
+    amb_model model;
+    uint64_t seed;
+} amb_realisation;
+
+typedef struct amb_batch amb_batch;
+
+/* fields[i] points at ny_i*nx_i doubles.  precision: AMB_F32 or AMB_F64. */
+int amb_batch_create(const amb_realisation* desc, int64_t count, const double* const* fields,
+                     int precision, amb_batch** out);
+int amb_batch_destroy(amb_batch* b);
+/* Advances every realisation by nsteps in hipGraph-captured launches. */
+int amb_batch_run(amb_batch* b, int64_t nsteps);
+int amb_batch_run_timed(amb_batch* b, int64_t nsteps, double* device_ms);
+int amb_batch_get_field(amb_batch* b, int64_t index, double* out);
+int amb_batch_mass(amb_batch* b, int64_t index, double* sum, int64_t* nonfinite);
+
+/* Tile shape (index from amb_tile_count) per realisation; realisations sharing a
+ * shape share one launch per step. */
+int amb_batch_set_tiles(amb_batch* b, const int* tile_per_realisation);
+int amb_batch_get_tiles(amb_batch* b, int* tile_per_realisation);
+
+/* Autotuner: for every distinct grid size, times `reps` steps of every compiled
+ * tile shape and keeps the fastest (state and step counter are restored).
+ * `chosen` (nullable) receives the tile index per realisation. */
+int amb_batch_autotune(amb_batch* b, int reps, int* chosen);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AMB_H */
