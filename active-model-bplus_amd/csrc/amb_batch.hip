@@ -32,8 +32,8 @@ struct BatchDesc {
     StepCoef<T> k;
 };
 
-template <typename T, int TW, int TH>
-__global__ __launch_bounds__(kThreads) void batch_step_kernel(const BatchDesc<T>* __restrict__ desc,
+template <typename T, int TW, int TH, int NT>
+__global__ __launch_bounds__(NT) void batch_step_kernel(const BatchDesc<T>* __restrict__ desc,
                                                               const int2* __restrict__ tiles, const T* in, T* out,
                                                               const long long* step_base, int step_off)
 {
@@ -43,44 +43,59 @@ __global__ __launch_bounds__(kThreads) void batch_step_kernel(const BatchDesc<T>
     const BatchDesc<T>& d = desc[t.x];
     const int ty = t.y / d.ntx, tx = t.y - ty * d.ntx;
     const uint64_t step = (uint64_t)(*step_base) + (uint64_t)step_off;
-    step_tile<T, TW, TH, kNoiseRuntime>(reinterpret_cast<T*>(smem_raw), in + d.off, out + d.off, d.ny, d.nx, d.nx4,
-                                        tx * TW, ty * TH, d.k, d.key0, d.key1, step, nullptr);
+    step_tile<T, TW, TH, NT, kNoiseRuntime>(reinterpret_cast<T*>(smem_raw), in + d.off, out + d.off, d.ny, d.nx,
+                                            d.nx4, tx * TW, ty * TH, d.k, d.key0, d.key1, step, nullptr);
 }
 
-template <typename T, int TW, int TH>
+template <typename T, int TW, int TH, int NT>
+static hipError_t raise_batch_lds()
+{
+    using L = Lds<T, TW, TH, NT>;
+    if (L::bytes <= 65536) return hipSuccess;
+    return hipFuncSetAttribute((const void*)batch_step_kernel<T, TW, TH, NT>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)L::bytes);
+}
+
+#define AMB_BRAISE_F32(i, tw, th, nt) if ((e = raise_batch_lds<float, tw, th, nt>()) != hipSuccess) return e;
+#define AMB_BRAISE_F64(i, tw, th, nt) if ((e = raise_batch_lds<double, tw, th, nt>()) != hipSuccess) return e;
+
+// Raises the dynamic-LDS limit of every batched instantiation (before any capture).
+static hipError_t prepare_batch_kernels()
+{
+    hipError_t e = hipSuccess;
+    AMB_F32_TILES(AMB_BRAISE_F32)
+    AMB_F64_TILES(AMB_BRAISE_F64)
+    return e;
+}
+
+template <typename T, int TW, int TH, int NT>
 static hipError_t launch_batch_one(const void* desc, const int2* tiles, int ntiles, const void* in, void* out,
                                    const long long* step, int step_off, hipStream_t s)
 {
-    using L = Lds<T, TW, TH>;
-    static bool raised = false;
-    if (!raised && L::bytes > 65536) {
-        const hipError_t e = hipFuncSetAttribute((const void*)batch_step_kernel<T, TW, TH>,
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)L::bytes);
-        if (e != hipSuccess) return e;
-        raised = true;
-    }
-    hipLaunchKernelGGL((batch_step_kernel<T, TW, TH>), dim3(ntiles), dim3(kThreads), L::bytes, s,
+    using L = Lds<T, TW, TH, NT>;
+    hipLaunchKernelGGL((batch_step_kernel<T, TW, TH, NT>), dim3(ntiles), dim3(NT), L::bytes, s,
                        static_cast<const BatchDesc<T>*>(desc), tiles, static_cast<const T*>(in), static_cast<T*>(out),
                        step, step_off);
     return hipGetLastError();
 }
+
+#define AMB_BCASE_F32(i, tw, th, nt) \
+    case i: return launch_batch_one<float, tw, th, nt>(desc, tiles, ntiles, in, out, step, step_off, s);
+#define AMB_BCASE_F64(i, tw, th, nt) \
+    case i: return launch_batch_one<double, tw, th, nt>(desc, tiles, ntiles, in, out, step, step_off, s);
 
 static hipError_t launch_batch(int precision, int tile, const void* desc, const int2* tiles, int ntiles,
                                const void* in, void* out, const long long* step, int step_off, hipStream_t s)
 {
     if (precision == AMB_F32) {
         switch (tile) {
-        case 0: return launch_batch_one<float, 64, 32>(desc, tiles, ntiles, in, out, step, step_off, s);
-        case 1: return launch_batch_one<float, 64, 16>(desc, tiles, ntiles, in, out, step, step_off, s);
-        case 2: return launch_batch_one<float, 32, 32>(desc, tiles, ntiles, in, out, step, step_off, s);
-        default: return launch_batch_one<float, 64, 64>(desc, tiles, ntiles, in, out, step, step_off, s);
+            AMB_F32_TILES(AMB_BCASE_F32)
+        default: return hipErrorInvalidValue;
         }
     }
     switch (tile) {
-    case 0: return launch_batch_one<double, 32, 32>(desc, tiles, ntiles, in, out, step, step_off, s);
-    case 1: return launch_batch_one<double, 64, 16>(desc, tiles, ntiles, in, out, step, step_off, s);
-    case 2: return launch_batch_one<double, 32, 16>(desc, tiles, ntiles, in, out, step, step_off, s);
-    default: return launch_batch_one<double, 64, 32>(desc, tiles, ntiles, in, out, step, step_off, s);
+        AMB_F64_TILES(AMB_BCASE_F64)
+    default: return hipErrorInvalidValue;
     }
 }
 
@@ -355,6 +370,7 @@ int amb_batch_create(const amb_realisation* desc, int64_t count, const double* c
         }                                         \
     } while (0)
     const size_t desc_bytes = (size_t)count * (precision == AMB_F32 ? sizeof(BatchDesc<float>) : sizeof(BatchDesc<double>));
+    AMB_CHECK(prepare_batch_kernels());
     AMB_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     AMB_CHECK(hipMalloc(&b->buf[0], (size_t)total * b->esize));
     AMB_CHECK(hipMalloc(&b->buf[1], (size_t)total * b->esize));

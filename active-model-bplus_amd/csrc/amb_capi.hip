@@ -79,9 +79,11 @@ StepCoef<T> make_coef(const amb_stencil& st, const amb_model& m)
         k.px[j] = (T)(-idx2 * kD8D8[j]);
         k.py[j] = (T)(-idy2 * kD8D8[j]);
     }
+    // G3xoG3x: the smoothing weights across rows are folded in (lxx[|a|]); for
+    // G3yoG3y the smoothing is applied to each row first, so lyy[0] = -1/dy^2.
     for (int a = 0; a < 3; ++a) {
         k.lxx[a] = (T)(-idx2 * kG3G3S[a]);
-        k.lyy[a] = (T)(-idy2 * kG3G3S[a]);
+        k.lyy[a] = (T)(a == 0 ? -idy2 : 0.0);
     }
     k.zx = (T)idx;
     k.zy = (T)idy;
@@ -115,6 +117,25 @@ int validate_params(int64_t ny, int64_t nx, const amb_stencil* st, const amb_mod
 }
 
 int grid_tiles(int64_t n, int t) { return (int)((n + t - 1) / t); }
+
+// Default tile: the first shape (fastest first) that still gives >= 4 workgroups per
+// CU of a 256-CU device; small fields fall back to the shape with the most tiles.
+int default_tile(int precision, int64_t ny, int64_t nx)
+{
+    const int n = tile_count(precision);
+    int most = 0;
+    int64_t most_tiles = 0;
+    for (int i = 0; i < n; ++i) {
+        const TileShape t = tile_shape(precision, i);
+        const int64_t tiles = (int64_t)grid_tiles(nx, t.tw) * grid_tiles(ny, t.th);
+        if (tiles >= 1024) return i;
+        if (tiles > most_tiles) {
+            most_tiles = tiles;
+            most = i;
+        }
+    }
+    return most;
+}
 
 // XCD-contiguous tile order: gfx950 dispatches workgroup i to XCD i % 8.
 int xcd_chunk(int nblocks) { return (nblocks % 8 == 0 && nblocks >= 64) ? nblocks / 8 : 0; }
@@ -359,13 +380,14 @@ int amb_tile_count(int precision, int* count)
     return AMB_OK;
 }
 
-int amb_tile_shape(int precision, int index, int* tile_cols, int* tile_rows)
+int amb_tile_shape(int precision, int index, int* tile_cols, int* tile_rows, int* threads)
 {
     if (precision != AMB_F32 && precision != AMB_F64) return fail(AMB_E_ARG, "bad precision");
     if (index < 0 || index >= tile_count(precision)) return fail(AMB_E_ARG, "tile index out of range");
     const TileShape t = tile_shape(precision, index);
     if (tile_cols) *tile_cols = t.tw;
     if (tile_rows) *tile_rows = t.th;
+    if (threads) *threads = t.nt;
     return AMB_OK;
 }
 
@@ -395,7 +417,7 @@ int amb_create(const double* field, int64_t ny, int64_t nx, const amb_stencil* s
     h->precision = opt.precision;
     h->esize = opt.precision == AMB_F32 ? 4 : 8;
     h->timestep = opt.timestep;
-    h->tile = opt.tile > 0 ? opt.tile - 1 : 0;
+    h->tile = opt.tile > 0 ? opt.tile - 1 : default_tile(opt.precision, ny, nx);
     const size_t n = (size_t)(ny * nx);
     hipError_t e;
 #define AMB_CHECK(call)                                  \

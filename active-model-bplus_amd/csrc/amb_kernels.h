@@ -15,7 +15,7 @@ struct StepCoef {
     T lcc, lex, ley, lck;  // L3: centre, x-neighbours, y-neighbours, corners
     T p0;                  // D8oD8 centre weight of -(mu_p'' along x and y)
     T px[9], py[9];        // D8oD8 weights (index 1..8) times -1/dx^2, -1/dy^2
-    T lxx[3], lyy[3];      // G3oG3 smoothing weights times -1/dx^2, -1/dy^2
+    T lxx[3], lyy[3];      // G3xoG3x row weights times -1/dx^2; lyy[0] = -1/dy^2
     T zx, zy;              // 1/dx, 1/dy (divergence of the circulating current)
     T nxw[5], nyw[5];      // D8 weights (index 1..4) times amp/dx, amp/dy (noise)
     T dt;
@@ -41,8 +41,14 @@ struct StepArgs {
 enum NoiseMode { kNoiseNone = 0, kNoisePhilox = 1, kNoiseBuffer = 2, kNoiseRuntime = 3 };
 
 struct TileShape {
-    int tw, th;
+    int tw, th, nt;  // tile columns, tile rows, workgroup size
 };
+
+// Compiled tile configurations (index = autotuner choice): X(index, TW, TH, NT).
+// Index 0 is the fastest measured on MI355X at 4096^2; the default picks the first
+// entry that still gives at least four workgroups per CU.
+#define AMB_F32_TILES(X) X(0, 32, 32, 256) X(1, 64, 32, 512) X(2, 64, 16, 256) X(3, 64, 64, 1024) X(4, 64, 32, 256) X(5, 32, 16, 256)
+#define AMB_F64_TILES(X) X(0, 32, 32, 256) X(1, 64, 16, 256) X(2, 32, 16, 256) X(3, 32, 32, 512)
 
 // Tile shapes compiled per precision (index = autotuner choice).
 int tile_count(int precision);

@@ -26,8 +26,8 @@ namespace amb {
 
 // ---- single-field kernel ----------------------------------------------------------
 
-template <typename T, int TW, int TH, int NOISE>
-__global__ __launch_bounds__(kThreads) void step_kernel(const StepArgs<T> p)
+template <typename T, int TW, int TH, int NT, int NOISE>
+__global__ __launch_bounds__(NT) void step_kernel(const StepArgs<T> p)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     // Tile index; with xcd_chunk the round-robin XCD dispatch gets contiguous tiles.
@@ -35,15 +35,16 @@ __global__ __launch_bounds__(kThreads) void step_kernel(const StepArgs<T> p)
     if (p.xcd_chunk > 0) t = (t & 7) * p.xcd_chunk + (t >> 3);
     const int ty = t / p.ntx, tx = t - ty * p.ntx;
     const uint64_t step = NOISE == kNoisePhilox ? (uint64_t)(*p.step_base) + (uint64_t)p.step_off : 0;
-    step_tile<T, TW, TH, NOISE>(reinterpret_cast<T*>(smem_raw), p.in, p.out, p.ny, p.nx, p.nx4, tx * TW, ty * TH, p.k,
-                                p.key0, p.key1, step, p.noise);
+    step_tile<T, TW, TH, NT, NOISE>(reinterpret_cast<T*>(smem_raw), p.in, p.out, p.ny, p.nx, p.nx4, tx * TW, ty * TH,
+                                    p.k, p.key0, p.key1, step, p.noise);
 }
 
 // ---- tile table ---------------------------------------------------------------
 
-// FP32 and FP64 tile shapes (autotuner search space).
-static const TileShape kTilesF32[] = {{64, 32}, {64, 16}, {32, 32}, {64, 64}};
-static const TileShape kTilesF64[] = {{32, 32}, {64, 16}, {32, 16}, {64, 32}};
+#define AMB_SHAPE(i, tw, th, nt) {tw, th, nt},
+static const TileShape kTilesF32[] = {AMB_F32_TILES(AMB_SHAPE)};
+static const TileShape kTilesF64[] = {AMB_F64_TILES(AMB_SHAPE)};
+#undef AMB_SHAPE
 
 int tile_count(int precision)
 {
@@ -56,74 +57,71 @@ TileShape tile_shape(int precision, int index)
     return precision == 32 ? kTilesF32[index] : kTilesF64[index];
 }
 
-template <typename T, int TW, int TH, int NOISE>
+template <typename T, int TW, int TH, int NT, int NOISE>
 static hipError_t launch_one(const StepArgs<T>& a, hipStream_t s)
 {
-    using L = Lds<T, TW, TH>;
+    using L = Lds<T, TW, TH, NT>;
     const int nblocks = a.ntx * a.nty;
-    hipLaunchKernelGGL((step_kernel<T, TW, TH, NOISE>), dim3(nblocks), dim3(kThreads), L::bytes, s, a);
+    hipLaunchKernelGGL((step_kernel<T, TW, TH, NT, NOISE>), dim3(nblocks), dim3(NT), L::bytes, s, a);
     return hipGetLastError();
 }
 
-template <typename T, int TW, int TH>
+template <typename T, int TW, int TH, int NT>
 static hipError_t launch_noise(const StepArgs<T>& a, int noise, hipStream_t s)
 {
     switch (noise) {
-    case kNoiseNone: return launch_one<T, TW, TH, kNoiseNone>(a, s);
-    case kNoisePhilox: return launch_one<T, TW, TH, kNoisePhilox>(a, s);
-    default: return launch_one<T, TW, TH, kNoiseBuffer>(a, s);
+    case kNoiseNone: return launch_one<T, TW, TH, NT, kNoiseNone>(a, s);
+    case kNoisePhilox: return launch_one<T, TW, TH, NT, kNoisePhilox>(a, s);
+    default: return launch_one<T, TW, TH, NT, kNoiseBuffer>(a, s);
     }
 }
+
+#define AMB_CASE_F32(i, tw, th, nt) case i: return launch_noise<float, tw, th, nt>(a, noise, s);
+#define AMB_CASE_F64(i, tw, th, nt) case i: return launch_noise<double, tw, th, nt>(a, noise, s);
 
 hipError_t launch_step_f32(const StepArgs<float>& a, int tile, int noise, hipStream_t s)
 {
     switch (tile) {
-    case 0: return launch_noise<float, 64, 32>(a, noise, s);
-    case 1: return launch_noise<float, 64, 16>(a, noise, s);
-    case 2: return launch_noise<float, 32, 32>(a, noise, s);
-    default: return launch_noise<float, 64, 64>(a, noise, s);
+        AMB_F32_TILES(AMB_CASE_F32)
+    default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_step_f64(const StepArgs<double>& a, int tile, int noise, hipStream_t s)
 {
     switch (tile) {
-    case 0: return launch_noise<double, 32, 32>(a, noise, s);
-    case 1: return launch_noise<double, 64, 16>(a, noise, s);
-    case 2: return launch_noise<double, 32, 16>(a, noise, s);
-    default: return launch_noise<double, 64, 32>(a, noise, s);
+        AMB_F64_TILES(AMB_CASE_F64)
+    default: return hipErrorInvalidValue;
     }
 }
 
-template <typename T, int TW, int TH>
+template <typename T, int TW, int TH, int NT>
 static hipError_t raise_lds_limit()
 {
-    using L = Lds<T, TW, TH>;
+    using L = Lds<T, TW, TH, NT>;
     hipError_t e = hipSuccess;
     if (L::bytes > 65536) {
-        e = hipFuncSetAttribute((const void*)step_kernel<T, TW, TH, kNoiseNone>,
+        e = hipFuncSetAttribute((const void*)step_kernel<T, TW, TH, NT, kNoiseNone>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)L::bytes);
         if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void*)step_kernel<T, TW, TH, kNoisePhilox>,
+            e = hipFuncSetAttribute((const void*)step_kernel<T, TW, TH, NT, kNoisePhilox>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)L::bytes);
         if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void*)step_kernel<T, TW, TH, kNoiseBuffer>,
+            e = hipFuncSetAttribute((const void*)step_kernel<T, TW, TH, NT, kNoiseBuffer>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)L::bytes);
     }
     return e;
 }
 
+#define AMB_RAISE_F32(i, tw, th, nt) if ((e = raise_lds_limit<float, tw, th, nt>()) != hipSuccess) return e;
+#define AMB_RAISE_F64(i, tw, th, nt) if ((e = raise_lds_limit<double, tw, th, nt>()) != hipSuccess) return e;
+
 hipError_t prepare_step_kernels()
 {
-    hipError_t e;
-    if ((e = raise_lds_limit<float, 64, 32>()) != hipSuccess) return e;
-    if ((e = raise_lds_limit<float, 64, 16>()) != hipSuccess) return e;
-    if ((e = raise_lds_limit<float, 32, 32>()) != hipSuccess) return e;
-    if ((e = raise_lds_limit<float, 64, 64>()) != hipSuccess) return e;
-    if ((e = raise_lds_limit<double, 32, 32>()) != hipSuccess) return e;
-    if ((e = raise_lds_limit<double, 64, 16>()) != hipSuccess) return e;
-    if ((e = raise_lds_limit<double, 32, 16>()) != hipSuccess) return e;
-    return raise_lds_limit<double, 64, 32>();
+    hipError_t e = hipSuccess;
+    AMB_F32_TILES(AMB_RAISE_F32)
+    AMB_F64_TILES(AMB_RAISE_F64)
+    return e;
 }
 
 // ---- step counter -------------------------------------------------------------

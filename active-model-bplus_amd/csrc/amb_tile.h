@@ -19,7 +19,7 @@ __device__ __forceinline__ int wrap_index(int g, int n)
     return g;
 }
 
-template <typename T, int TW, int TH>
+template <typename T, int TW, int TH, int NT>
 struct Lds {
     static constexpr int PW = TW + 18, PH = TH + 18;  // phi: rows [-9,TH+9), cols [-9,TW+9)
     static constexpr int MW = TW + 16, MH = TH + 16;  // mu_p: rows [-8,TH+8), cols [-8,TW+8)
@@ -36,9 +36,9 @@ struct Lds {
     static constexpr int off_qy = off_qx + QW * QH;
     static constexpr int total = off_qy + QW * QH;
     static constexpr size_t bytes = sizeof(T) * (size_t)total;
-    static constexpr int rows_per_thread = TW * TH / kThreads;
-    static_assert(kThreads % TW == 0, "tile width must divide the workgroup");
-    static_assert((TW * TH) % kThreads == 0, "tile must be a multiple of the workgroup");
+    static_assert(NT % TW == 0, "tile width must divide the workgroup");
+    static_assert((TW * TH) % NT == 0, "tile must be a multiple of the workgroup");
+    static_assert(NT >= MW, "workgroup must cover one stage-A row of the halo region");
     static_assert(TW % 4 == 0, "noise column groups need TW % 4 == 0");
 };
 
@@ -87,13 +87,13 @@ __device__ __forceinline__ T normal1(int row, int col, uint64_t step, int comp, 
 
 // ---- fused step --------------------------------------------------------------
 
-template <typename T, int TW, int TH, int NOISE>
+template <typename T, int TW, int TH, int NT, int NOISE>
 __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in, T* __restrict__ out, const int ny,
                                           const int nx, const int nx4, const int x0, const int y0,
                                           const StepCoef<T>& K, const uint32_t key0, const uint32_t key1,
                                           const uint64_t step, const double* noise)
 {
-    using L = Lds<T, TW, TH>;
+    using L = Lds<T, TW, TH, NT>;
     T* const s_phi = sm;
     T* const s_mup = sm + L::off_mup;
     T* const s_mua = sm + L::off_mua;
@@ -103,127 +103,253 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
     T* const s_xiy = sm + L::XW * L::XH;
 
     const int tid = threadIdx.x;
-
-    // stage 0: phi with a 9-site halo
-    for (int i = tid; i < L::n_phi; i += kThreads) {
-        const int r = i / L::PW, c = i - r * L::PW;
-        const int gr = wrap_index(y0 - 9 + r, ny), gc = wrap_index(x0 - 9 + c, nx);
-        s_phi[i] = in[(size_t)gr * nx + gc];
-    }
-    __syncthreads();
-
-    constexpr int R = L::rows_per_thread;
+    constexpr int RG = NT / TW;        // row groups of stage B
+    constexpr int R = TH / RG;         // output rows per thread in stage B
+    static_assert(TH % RG == 0, "tile height must split evenly over the row groups");
     const int cx = tid % TW;
-    const int ry = (tid / TW) * R;
-    T phi_c[R];
-#pragma unroll
-    for (int k = 0; k < R; ++k) phi_c[k] = s_phi[(ry + k + 9) * L::PW + cx + 9];
+    const int rg = tid / TW;
 
-    // stage A: local terms on the halo region (corners beyond the cross skipped)
-    for (int i = tid; i < L::MW * L::MH; i += kThreads) {
-        const int rr = i / L::MW, cc = i - rr * L::MW;
-        const int r = rr - 8, c = cc - 8;
-        const bool rin = (unsigned)r < (unsigned)TH, cin = (unsigned)c < (unsigned)TW;
-        if (!(rin || cin)) continue;
-        const T* q = s_phi + (r + 9) * L::PW + (c + 9);
-        const T pmm = q[-L::PW - 1], pm0 = q[-L::PW], pmp = q[-L::PW + 1];
-        const T p0m = q[-1], p00 = q[0], p0p = q[1];
-        const T ppm = q[L::PW - 1], pp0 = q[L::PW], ppp = q[L::PW + 1];
-        const T gx = K.gxe * ((pmp - pmm) + (ppp - ppm)) + K.gxc * (p0p - p0m);
-        const T gy = K.gye * ((ppm - pmm) + (ppp - pmp)) + K.gyc * (pp0 - pm0);
-        const T lap = K.lcc * p00 + K.lex * (p0m + p0p) + K.ley * (pm0 + pp0) +
-                      K.lck * ((pmm + pmp) + (ppm + ppp));
-        s_mup[i] = p00 * (K.a + p00 * (K.b + p00 * K.c)) - K.kappa * lap;
-        if ((unsigned)(r + 2) < (unsigned)(TH + 4) && (unsigned)(c + 2) < (unsigned)(TW + 4))
-            s_mua[(r + 2) * L::AW + (c + 2)] = K.lam * (gy * gy + gx * gx);
-        if ((unsigned)(r + 1) < (unsigned)(TH + 2) && (unsigned)(c + 1) < (unsigned)(TW + 2)) {
-            const T zl = K.zeta * lap;
-            s_qx[(r + 1) * L::QW + (c + 1)] = zl * gx;
-            s_qy[(r + 1) * L::QW + (c + 1)] = zl * gy;
+    // Interior tiles (halo inside the field) use affine addressing; tiles on the
+    // periodic seam or past the edge of the field wrap every index.
+    const bool interior = x0 >= 9 && x0 + TW + 9 <= nx && y0 >= 9 && y0 + TH + 9 <= ny;
+
+    // ---- stage 0: phi with a 9-site halo -> LDS.  Each thread owns fixed region
+    // columns and walks rows; loads are independent and unrolled.
+    {
+        constexpr int NCJ = (L::PW + TW - 1) / TW;
+        if (interior) {
+            const T* src = in + (size_t)(y0 - 9 + rg) * nx + (x0 - 9 + cx);
+#pragma unroll
+            for (int r = rg; r < L::PH; r += RG) {
+#pragma unroll
+                for (int j = 0; j < NCJ; ++j)
+                    if (j < NCJ - 1 || cx + j * TW < L::PW) s_phi[r * L::PW + cx + j * TW] = src[j * TW];
+                src += (size_t)RG * nx;
+            }
+        } else {
+            int gcol[NCJ];
+#pragma unroll
+            for (int j = 0; j < NCJ; ++j) gcol[j] = wrap_index(x0 - 9 + cx + j * TW, nx);
+#pragma unroll
+            for (int r = rg; r < L::PH; r += RG) {
+                const T* row = in + (size_t)wrap_index(y0 - 9 + r, ny) * nx;
+#pragma unroll
+                for (int j = 0; j < NCJ; ++j)
+                    if (j < NCJ - 1 || cx + j * TW < L::PW) s_phi[r * L::PW + cx + j * TW] = row[gcol[j]];
+            }
         }
     }
     __syncthreads();
 
-    // stage N: noise normals (overwrites the phi buffer)
+    T phi_c[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) phi_c[k] = s_phi[(rg * R + k + 9) * L::PW + cx + 9];
+
+    // ---- stage A: local terms on the (TH+16) x (TW+16) halo region, marched down
+    // column strips so each phi row is read once and its horizontal difference and
+    // sum are reused by the three output rows that need them.
+    {
+        constexpr int NSEG = NT / L::MW;
+        constexpr int SEG = (L::MH + NSEG - 1) / NSEG;
+        if (tid < NSEG * L::MW) {
+            const int seg = tid / L::MW;
+            const int cc = tid - seg * L::MW;  // region column, c = cc - 8
+            const int rr0 = seg * SEG;         // region row, r = rr - 8
+            const T* P = s_phi + rr0 * L::PW + cc;  // phi at (r0 - 1, c - 1)
+            T tm = P[0], t0 = P[1], tp = P[2];
+            T mm = P[L::PW], m0 = P[L::PW + 1], mp = P[L::PW + 2];
+            T th1 = tp - tm, th2 = tp + tm;
+            T mh1 = mp - mm, mh2 = mp + mm;
+            const int c = cc - 8;
+            const bool cbox2 = (unsigned)(c + 2) < (unsigned)(TW + 4);
+            const bool cbox1 = (unsigned)(c + 1) < (unsigned)(TW + 2);
+#pragma unroll
+            for (int k = 0; k < SEG; ++k) {
+                const int rr = rr0 + k;
+                if (rr < L::MH) {
+                    const T* Q = P + (k + 2) * L::PW;
+                    const T bm = Q[0], b0 = Q[1], bp = Q[2];
+                    const T bh1 = bp - bm, bh2 = bp + bm;
+                    const T gx = K.gxe * (th1 + bh1) + K.gxc * mh1;
+                    const T gy = K.gye * (bh2 - th2) + K.gyc * (b0 - t0);
+                    const T lap = K.lcc * m0 + K.lex * mh2 + K.ley * (t0 + b0) + K.lck * (th2 + bh2);
+                    s_mup[rr * L::MW + cc] = m0 * (K.a + m0 * (K.b + m0 * K.c)) - K.kappa * lap;
+                    const int r = rr - 8;
+                    if (cbox2 && (unsigned)(r + 2) < (unsigned)(TH + 4))
+                        s_mua[(r + 2) * L::AW + (c + 2)] = K.lam * (gy * gy + gx * gx);
+                    if (cbox1 && (unsigned)(r + 1) < (unsigned)(TH + 2)) {
+                        const T zl = K.zeta * lap;
+                        s_qx[(r + 1) * L::QW + (c + 1)] = zl * gx;
+                        s_qy[(r + 1) * L::QW + (c + 1)] = zl * gy;
+                    }
+                    tm = mm; t0 = m0; tp = mp; th1 = mh1; th2 = mh2;
+                    mm = bm; m0 = b0; mp = bp; mh1 = bh1; mh2 = bh2;
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- stage N: noise normals (overwrites the phi buffer)
     bool noisy = NOISE == kNoisePhilox || NOISE == kNoiseBuffer;
     if constexpr (NOISE == kNoiseRuntime) noisy = K.amp != T(0);
     if (NOISE == kNoisePhilox || (NOISE == kNoiseRuntime && noisy)) {
-        constexpr int GXN = TW / 4 + 2;  // xi_x column groups: tile-relative [-1, TW/4]
-        for (int i = tid; i < GXN * TH; i += kThreads) {
-            const int r = i / GXN, gl = i - r * GXN - 1;
-            const int grow = wrap_index(y0 + r, ny);
-            T* dst = s_xix + r * L::XW + 4 * (gl + 1);
-            if (nx4) {
-                const int gc = wrap_index(x0 + 4 * gl, nx);
+        // One flat list of Philox calls: xi_x groups (rows [0,TH), tile-relative column
+        // groups [-1, TW/4]) then xi_y groups (rows [-4,TH+4), groups [0, TW/4)).  Each
+        // thread runs PER independent calls, unrolled so their multiply chains overlap.
+        constexpr int GXN = TW / 4 + 2, GYN = TW / 4;
+        constexpr int NXI = GXN * TH, NYI = GYN * L::YH, NITEMS = NXI + NYI;
+        constexpr int PERX = (NXI + NT - 1) / NT, PERY = (NYI + NT - 1) / NT;
+        if (nx4 && interior) {
+            // xi_x: rows [0, TH), column groups x0/4 - 1 .. x0/4 + TW/4
+#pragma unroll
+            for (int u = 0; u < PERX; ++u) {
+                const int i0 = tid + u * NT;
+                const int i = i0 < NXI ? i0 : NXI - 1;
+                const int r = i / GXN, gl = i - r * GXN;
                 T z[4];
-                normals4<T>((uint32_t)(gc >> 2), (uint32_t)grow, step, 1, key0, key1, z);
-                dst[0] = z[0]; dst[1] = z[1]; dst[2] = z[2]; dst[3] = z[3];
-            } else {
-                for (int l = 0; l < 4; ++l)
-                    dst[l] = normal1<T>(grow, wrap_index(x0 + 4 * gl + l, nx), step, 1, key0, key1);
+                normals4<T>((uint32_t)((x0 >> 2) + gl - 1), (uint32_t)(y0 + r), step, 1, key0, key1, z);
+                if (i0 < NXI) {
+                    T* dst = s_xix + r * L::XW + 4 * gl;
+                    dst[0] = z[0]; dst[1] = z[1]; dst[2] = z[2]; dst[3] = z[3];
+                }
             }
-        }
-        constexpr int GYN = TW / 4;  // xi_y column groups [0, TW/4), rows [-4, TH+4)
-        for (int i = tid; i < GYN * L::YH; i += kThreads) {
-            const int rr = i / GYN, gl = i - rr * GYN;
-            const int grow = wrap_index(y0 + rr - 4, ny);
-            T z[4];
-            normals4<T>((uint32_t)((x0 >> 2) + gl), (uint32_t)grow, step, 0, key0, key1, z);
-            T* dst = s_xiy + rr * L::YW + 4 * gl;
-            dst[0] = z[0]; dst[1] = z[1]; dst[2] = z[2]; dst[3] = z[3];
+            // xi_y: rows [-4, TH+4), column groups x0/4 .. x0/4 + TW/4 - 1
+#pragma unroll
+            for (int u = 0; u < PERY; ++u) {
+                const int i0 = tid + u * NT;
+                const int i = i0 < NYI ? i0 : NYI - 1;
+                const int rr = i / GYN, gl = i - rr * GYN;
+                T z[4];
+                normals4<T>((uint32_t)((x0 >> 2) + gl), (uint32_t)(y0 + rr - 4), step, 0, key0, key1, z);
+                if (i0 < NYI) {
+                    T* dst = s_xiy + rr * L::YW + 4 * gl;
+                    dst[0] = z[0]; dst[1] = z[1]; dst[2] = z[2]; dst[3] = z[3];
+                }
+            }
+        } else if (nx4) {
+            constexpr int PER = (NITEMS + NT - 1) / NT;
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                const int i0 = tid + u * NT;
+                const int i = i0 < NITEMS ? i0 : NITEMS - 1;
+                uint32_t group, grow;
+                int comp;
+                T* dst;
+                if (i < NXI) {
+                    const int r = i / GXN, gl = i - r * GXN - 1;
+                    grow = (uint32_t)wrap_index(y0 + r, ny);
+                    group = (uint32_t)(wrap_index(x0 + 4 * gl, nx) >> 2);
+                    comp = 1;
+                    dst = s_xix + r * L::XW + 4 * (gl + 1);
+                } else {
+                    const int j = i - NXI;
+                    const int rr = j / GYN, gl = j - rr * GYN;
+                    grow = (uint32_t)wrap_index(y0 + rr - 4, ny);
+                    group = (uint32_t)((x0 >> 2) + gl);
+                    comp = 0;
+                    dst = s_xiy + rr * L::YW + 4 * gl;
+                }
+                T z[4];
+                normals4<T>(group, grow, step, comp, key0, key1, z);
+                if (i0 < NITEMS) {
+                    dst[0] = z[0]; dst[1] = z[1]; dst[2] = z[2]; dst[3] = z[3];
+                }
+            }
+        } else {
+            // Column groups straddle the periodic seam: one call per site.
+            for (int i = tid; i < NITEMS; i += NT) {
+                if (i < NXI) {
+                    const int r = i / GXN, gl = i - r * GXN - 1;
+                    const int grow = wrap_index(y0 + r, ny);
+                    T* dst = s_xix + r * L::XW + 4 * (gl + 1);
+                    for (int l = 0; l < 4; ++l)
+                        dst[l] = normal1<T>(grow, wrap_index(x0 + 4 * gl + l, nx), step, 1, key0, key1);
+                } else {
+                    const int j = i - NXI;
+                    const int rr = j / GYN, gl = j - rr * GYN;
+                    T z[4];
+                    normals4<T>((uint32_t)((x0 >> 2) + gl), (uint32_t)wrap_index(y0 + rr - 4, ny), step, 0, key0,
+                                key1, z);
+                    T* dst = s_xiy + rr * L::YW + 4 * gl;
+                    dst[0] = z[0]; dst[1] = z[1]; dst[2] = z[2]; dst[3] = z[3];
+                }
+            }
         }
         __syncthreads();
     } else if (NOISE == kNoiseBuffer) {
         const size_t n = (size_t)ny * nx;
-        for (int i = tid; i < L::XW * L::XH; i += kThreads) {
+        for (int i = tid; i < L::XW * L::XH; i += NT) {
             const int r = i / L::XW, c = i - r * L::XW;
             s_xix[i] = (T)noise[n + (size_t)wrap_index(y0 + r, ny) * nx + wrap_index(x0 + c - 4, nx)];
         }
-        for (int i = tid; i < L::YW * L::YH; i += kThreads) {
+        for (int i = tid; i < L::YW * L::YH; i += NT) {
             const int r = i / L::YW, c = i - r * L::YW;
             s_xiy[i] = (T)noise[(size_t)wrap_index(y0 + r - 4, ny) * nx + wrap_index(x0 + c, nx)];
         }
         __syncthreads();
     }
 
-    // stage B: divergence and update
+    // ---- stage B: one column strip of R output rows per thread.  Row-wise partial
+    // sums of the separable stencils are formed once per row and shared by the
+    // output rows that need them.
+    const int c = cx, r0 = rg * R;
+    T* orow = out + (size_t)(y0 + r0) * nx + (x0 + c);
+    T mcol[R + 16];  // mu_p at rows r0-8 .. r0+R+7 of this column
+#pragma unroll
+    for (int k = 0; k < R + 16; ++k) mcol[k] = s_mup[(r0 + k) * L::MW + c + 8];
+    T d2x[R + 4], sx[R + 4];  // mu_a rows r0-2 .. r0+R+1
+#pragma unroll
+    for (int k = 0; k < R + 4; ++k) {
+        const T* A = s_mua + (r0 + k) * L::AW + c + 2;
+        const T o2 = A[-2] + A[2];
+        d2x[k] = o2 - A[0] - A[0];                                            // (1,0,-2,0,1) along x
+        sx[k] = T(kG3G3S[0]) * A[0] + T(kG3G3S[1]) * (A[-1] + A[1]) + T(kG3G3S[2]) * o2;  // (.01,.06,.11,.06,.01)
+    }
+    T hx[R + 2], sy[R + 2];  // circulating current rows r0-1 .. r0+R
+#pragma unroll
+    for (int k = 0; k < R + 2; ++k) {
+        const T* Qx = s_qx + (r0 + k) * L::QW + c + 1;
+        const T* Qy = s_qy + (r0 + k) * L::QW + c + 1;
+        hx[k] = Qx[1] - Qx[-1];
+        sy[k] = K.gyc * Qy[0] + K.gye * (Qy[-1] + Qy[1]);
+    }
+    T ycol[NOISE == kNoiseNone ? 1 : R + 8];  // xi_y at rows r0-4 .. r0+R+3
+    if (noisy) {
+#pragma unroll
+        for (int k = 0; k < R + 8; ++k) ycol[k] = s_xiy[(r0 + k) * L::YW + c];
+    }
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-        const int r = ry + k, c = cx;
-        // passive: -(1/dy^2) D8oD8_y mu_p - (1/dx^2) D8oD8_x mu_p
-        const T* m = s_mup + (r + 8) * L::MW + (c + 8);
-        T acc = K.p0 * m[0];
+        const int r = r0 + k;
+        // passive: -(1/dx^2) D8oD8_x mu_p - (1/dy^2) D8oD8_y mu_p
+        const T* m = s_mup + (r + 8) * L::MW + c + 8;
+        T acc = K.p0 * mcol[k + 8];
 #pragma unroll
-        for (int j = 1; j <= 8; ++j)
-            acc += K.px[j] * (m[j] + m[-j]) + K.py[j] * (m[j * L::MW] + m[-j * L::MW]);
+        for (int j = 1; j <= 8; ++j) {
+            acc = fma(K.px[j], m[j] + m[-j], acc);
+            acc = fma(K.py[j], mcol[k + 8 + j] + mcol[k + 8 - j], acc);
+        }
         // lambda: -(1/dx^2) G3xoG3x mu_a - (1/dy^2) G3yoG3y mu_a
-        const T* A = s_mua + (r + 2) * L::AW + (c + 2);
-        T gxx = K.lxx[0] * (A[2] + A[-2] - A[0] - A[0]);
-        T gyy = K.lyy[0] * (A[2 * L::AW] + A[-2 * L::AW] - A[0] - A[0]);
-#pragma unroll
-        for (int a = 1; a <= 2; ++a) {
-            const int u = a * L::AW;
-            gxx += K.lxx[a] * ((A[u + 2] + A[u - 2] - A[u] - A[u]) + (A[-u + 2] + A[-u - 2] - A[-u] - A[-u]));
-            gyy += K.lyy[a] * ((A[2 * L::AW + a] + A[-2 * L::AW + a] - A[a] - A[a]) +
-                               (A[2 * L::AW - a] + A[-2 * L::AW - a] - A[-a] - A[-a]));
-        }
-        acc += gxx + gyy;
+        acc = fma(K.lxx[0], d2x[k + 2], acc);
+        acc = fma(K.lxx[1], d2x[k + 1] + d2x[k + 3], acc);
+        acc = fma(K.lxx[2], d2x[k] + d2x[k + 4], acc);
+        acc = fma(K.lyy[0], (sx[k] + sx[k + 4]) - sx[k + 2] - sx[k + 2], acc);
         // circulating: (1/dx) G3x q_x + (1/dy) G3y q_y
-        const T* Qx = s_qx + (r + 1) * L::QW + (c + 1);
-        const T* Qy = s_qy + (r + 1) * L::QW + (c + 1);
-        acc += K.gxc * (Qx[1] - Qx[-1]) + K.gxe * ((Qx[L::QW + 1] - Qx[L::QW - 1]) + (Qx[-L::QW + 1] - Qx[-L::QW - 1]));
-        acc += K.gyc * (Qy[L::QW] - Qy[-L::QW]) + K.gye * ((Qy[L::QW + 1] - Qy[-L::QW + 1]) + (Qy[L::QW - 1] - Qy[-L::QW - 1]));
+        acc = fma(K.gxc, hx[k + 1], acc);
+        acc = fma(K.gxe, hx[k] + hx[k + 2], acc);
+        acc += sy[k + 2] - sy[k];
         if (noisy) {
-            const T* X = s_xix + r * L::XW + (c + 4);
-            const T* Y = s_xiy + (r + 4) * L::YW + c;
-            T nsum = 0;
+            const T* X = s_xix + r * L::XW + c + 4;
 #pragma unroll
-            for (int j = 1; j <= 4; ++j)
-                nsum += K.nxw[j] * (X[j] - X[-j]) + K.nyw[j] * (Y[j * L::YW] - Y[-j * L::YW]);
-            acc += nsum;
+            for (int j = 1; j <= 4; ++j) {
+                acc = fma(K.nxw[j], X[j] - X[-j], acc);
+                acc = fma(K.nyw[j], ycol[k + 4 + j] - ycol[k + 4 - j], acc);
+            }
         }
-        const int gr = y0 + r, gc = x0 + c;
-        if (gr < ny && gc < nx) out[(size_t)gr * nx + gc] = phi_c[k] - K.dt * acc;
+        const T v = fma(-K.dt, acc, phi_c[k]);
+        if (interior || (y0 + r < ny && x0 + c < nx)) *orow = v;
+        orow += nx;
     }
 }
 

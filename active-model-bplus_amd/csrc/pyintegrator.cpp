@@ -14,6 +14,7 @@
 #include <random>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "amb.h"
@@ -427,11 +428,11 @@ PYBIND11_MODULE(integrator, m)
     m.def("tile_shapes", [](int precision) {
         int n = 0;
         check(amb_tile_count(precision, &n));
-        std::vector<std::pair<int, int>> out;
+        std::vector<std::tuple<int, int, int>> out;
         for (int i = 0; i < n; ++i) {
-            int tw = 0, th = 0;
-            check(amb_tile_shape(precision, i, &tw, &th));
-            out.emplace_back(tw, th);
+            int tw = 0, th = 0, nt = 0;
+            check(amb_tile_shape(precision, i, &tw, &th, &nt));
+            out.emplace_back(tw, th, nt);
         }
         return out;
     });

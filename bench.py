@@ -92,8 +92,9 @@ def bench_single(args, key):
 
     w = WORKLOADS[key]
     phi = initial_field(w)
-    sim = Integrator(phi, Stencil(*w["stencil"]), Model(*w["model"]), seed=w["seed"], precision=w["precision"])
-    tile = sim.autotune(4) if args.autotune else 0
+    sim = Integrator(phi, Stencil(*w["stencil"]), Model(*w["model"]), seed=w["seed"], precision=w["precision"],
+                     tile=args.tile)
+    tile = sim.autotune(4) if args.autotune else sim.options["tile"] - 1
     if args.warmup:
         sim.run(args.warmup)
     t0 = time.perf_counter()
@@ -113,7 +114,7 @@ def bench_single(args, key):
         "data": "synthetic (seeded numpy normal initial field)",
         "config": {"workload": w["label"], "grid": [w["n"], w["n"]], "stencil": w["stencil"],
                    "model": dict(zip(["a", "b", "c", "kappa", "lamb", "zeta", "T"], w["model"])),
-                   "tile": list(sim_tile_shape(w["precision"], tile))},
+                   "tile": dict(zip(["cols", "rows", "threads"], sim_tile_shape(w["precision"], tile)))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(f"step_{key}")},
         "device_ms_per_step": device_ms / args.steps,
@@ -184,21 +185,50 @@ def bench_batched(args):
     }
 
 
+def bench_copy(args):
+    """Plain device copy of a 4096 x 4096 FP32 array (read 64 MiB + write 64 MiB per
+    launch): the calibration kernel for the rocprofv3 FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    src = torch.randn(4096, 4096, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(max(1, args.warmup)):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    nbytes = 2 * src.numel() * 4
+    return {"metric": "copy GB/s (counter calibration)", "value": nbytes * args.steps / wall / 1e9, "unit": "GB/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic", "config": {"workload": "plain copy 4096x4096 FP32 (torch)",
+                                              "bytes_per_launch": nbytes}}
+
+
 def main():
     p = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=2000)
     p.add_argument("--warmup", type=int, default=64)
-    p.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c4"])
+    p.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c4", "copy"])
     p.add_argument("--realisations", type=int, default=512)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     p.add_argument("--autotune", action="store_true", help="pick the tile shape by timing")
+    p.add_argument("--tile", type=int, default=0, help="0 = default tile, k > 0 = tile index k-1")
     args = p.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU slab "
               "decomposition is a design, not a measured path.", file=sys.stderr)
         sys.exit(2)
-    out = bench_batched(args) if args.workload == "c4" else bench_single(args, args.workload)
+    if args.workload == "c4":
+        out = bench_batched(args)
+    elif args.workload == "copy":
+        out = bench_copy(args)
+    else:
+        out = bench_single(args, args.workload)
     print(json.dumps(out))
 
 

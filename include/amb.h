@@ -118,7 +118,7 @@ int amb_run_timed(amb_handle* h, int64_t nsteps, double* device_ms);
 
 /* Tile shapes compiled for the step kernel of a precision (autotuner search space). */
 int amb_tile_count(int precision, int* count);
-int amb_tile_shape(int precision, int index, int* tile_cols, int* tile_rows);
+int amb_tile_shape(int precision, int index, int* tile_cols, int* tile_rows, int* threads);
 int amb_set_tile(amb_handle* h, int index);
 
 /* Autotuner for one field: times `reps` steps of every compiled tile shape on a

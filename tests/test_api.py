@@ -77,7 +77,7 @@ def test_tile_tables():
     for precision in (32, 64):
         shapes = tile_shapes(precision)
         assert len(shapes) >= 3
-        assert all(tw % 4 == 0 and (tw * th) % 256 == 0 for tw, th in shapes)
+        assert all(tw % 4 == 0 and (tw * th) % nt == 0 and nt % 64 == 0 for tw, th, nt in shapes)
 
 
 def test_library_exports_every_declared_symbol():
