@@ -30,6 +30,7 @@ sys.path.insert(0, os.path.join(REPO, "active-model-bplus_amd"))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = {"c3": "step_c3_fp32_4096"}
 METRIC = "site-updates/s, % HBM peak; batched vs plain; surrogate speedup at ≤2% S(k) err"
 
 WORKLOADS = {
@@ -78,13 +79,21 @@ def cpu_baseline(w, phi, budget_s=15.0):
                       f"{threads} OpenMP threads, {dt:.1f} s"}
 
 
-def read_traffic(name):
-    """HBM bytes per launch measured by rocprofv3 --pmc (committed under profiles/), or None."""
-    path = os.path.join(REPO, "profiles", f"{name}_traffic.json")
+def read_traffic(name, tile):
+    """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and
+    calibrated against a plain copy kernel (profiles/<name>_traffic.md), or None when
+    no measurement exists for this workload and tile shape."""
+    path = os.path.join(REPO, "profiles", f"{name}_traffic.md")
     if not os.path.exists(path):
         return None
-    with open(path) as f:
-        return json.load(f).get("hbm_bytes_per_launch")
+    fields = {}
+    for line in open(path):
+        if ":" in line and line.split(":")[0].strip() in ("traffic_tile", "traffic_bytes_per_launch"):
+            k, v = line.split(":", 1)
+            fields[k.strip()] = v.strip()
+    if fields.get("traffic_tile") != "x".join(str(t) for t in tile):
+        return None
+    return int(fields["traffic_bytes_per_launch"])
 
 
 def bench_single(args, key):
@@ -116,7 +125,8 @@ def bench_single(args, key):
                    "model": dict(zip(["a", "b", "c", "kappa", "lamb", "zeta", "T"], w["model"])),
                    "tile": dict(zip(["cols", "rows", "threads"], sim_tile_shape(w["precision"], tile)))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(f"step_{key}")},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": read_traffic(TRAFFIC_PROFILE.get(key, ""), sim_tile_shape(w["precision"], tile))},
         "device_ms_per_step": device_ms / args.steps,
         "mass_fp64": mass, "nonfinite": bad,
     }
