@@ -23,9 +23,9 @@ import numpy as np
 from .integrator import CudaException, HipException, Model, Stencil
 from .integrator import Ensemble as _BaseEnsemble
 from .integrator import Integrator as BaseIntegrator
-from .integrator import device_count, tile_shapes
+from .integrator import Surrogate, device_count, tile_shapes
 
-__all__ = ["Model", "Stencil", "Integrator", "Ensemble", "CudaException", "HipException",
+__all__ = ["Model", "Stencil", "Integrator", "Ensemble", "Surrogate", "CudaException", "HipException",
            "steps_from_time", "step_iterator", "plot_field", "device_count", "tile_shapes"]
 
 

@@ -357,9 +357,42 @@ int compute_observables(amb_handle* h)
 
 }  // namespace
 
+namespace amb {
+
+int handle_view(amb_handle* h, FieldView* v)
+{
+    if (!h || !v) return fail(AMB_E_ARG, "handle must not be null");
+    v->cur = h->buf[h->cur];
+    v->other = h->buf[h->cur ^ 1];
+    v->ny = h->ny;
+    v->nx = h->nx;
+    v->precision = h->precision;
+    v->stream = h->stream;
+    return AMB_OK;
+}
+
+int handle_commit(amb_handle* h, int64_t steps, bool swapped)
+{
+    if (swapped) h->cur ^= 1;
+    h->timestep += steps;
+    const hipError_t e = launch_advance(h->d_step, steps, h->stream);
+    return e == hipSuccess ? AMB_OK : fail_hip(e, "advance launch");
+}
+
+}  // namespace amb
+
 extern "C" {
 
 const char* amb_last_error(void) { return g_error.c_str(); }
+
+int amb_set_field(amb_handle* h, const double* field)
+{
+    clear_error();
+    if (!h || !field) return fail(AMB_E_ARG, "handle and field must not be null");
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return fail_hip(e, "hipStreamSynchronize");
+    return upload_field(h, field, h->cur);
+}
 
 int amb_device_count(int* count)
 {

@@ -22,4 +22,17 @@ int xcd_chunk(int nblocks);
 template <typename T>
 StepCoef<T> make_coef(const amb_stencil& st, const amb_model& m);
 
+// Access to an integrator's state for other parts of the library (the surrogate).
+struct FieldView {
+    void* cur;          // current field (ny*nx elements of the handle's precision)
+    void* other;        // the second ping-pong buffer
+    int64_t ny, nx;
+    int precision;
+    hipStream_t stream;
+};
+int handle_view(amb_handle* h, FieldView* v);
+// Advances the step counter by `steps` (host and device); with `swapped` the field
+// now lives in the buffer that was `other`.
+int handle_commit(amb_handle* h, int64_t steps, bool swapped);
+
 }  // namespace amb

@@ -1,0 +1,1162 @@
+// amb_unet.hip -- learned flux-form surrogate: a small periodic U-Net whose
+// convolutions run on gfx950 BF16 matrix cores (v_mfma_f32_16x16x32_bf16, FP32
+// accumulate), with hand-written forward, backward and Adam kernels.  No reference
+// counterpart.  The network (3 levels, 32/64/128 channels, 3x3 kernels, GELU)
+// predicts a current J so that phi <- phi - div J: mass is conserved exactly by the
+// backward-difference divergence, whatever the network outputs.
+//
+// Layout: activations NHWC bfloat16 (pixel-major, channels contiguous), weights
+// [tap][cout][cin] bfloat16 (forward) and [tap][cin][cout] (backward data), FP32
+// master weights and Adam moments in one flat vector, tap = 3(dy+1) + (dx+1)
+// reading in[y+dy][x+dx] with periodic wrap.
+//
+// Convolution as implicit GEMM: M = pixels, N = output channels, K = taps x cin.
+// Each wave owns MB x NB tiles of 16 x 16 outputs; per K-step of 32 a lane loads one
+// 16-byte chunk (8 channels of one tap) per A tile and per B tile straight from
+// global memory (L1/L2 resident) into the MFMA operand layout:
+//   A: lane l holds A[l%16][8(l/16) .. +7];  B: B[8(l/16) .. +7][l%16];
+//   D: lane l holds D[4(l/16) + r][l%16], r = 0..3.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "amb.h"
+#include "amb_internal.h"
+
+namespace amb {
+namespace unet {
+
+typedef uint16_t bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kCin0 = 16;  // input channels: phi, lamb/2, zeta/5, phi0/0.6, 12 zero pads
+constexpr float kParamScale[3] = {2.0f, 5.0f, 0.6f};
+constexpr float kGeluC0 = 0.7978845608028654f, kGeluC1 = 0.044715f;
+
+__device__ __forceinline__ float bf2f(bf16 h) { return __uint_as_float((uint32_t)h << 16); }
+
+// float -> bfloat16, round to nearest even (NaN kept quiet).
+__host__ __device__ __forceinline__ bf16 f2bf(float f)
+{
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (bf16)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (bf16)(u >> 16);
+}
+
+__device__ __forceinline__ float gelu(float x)
+{
+    return 0.5f * x * (1.0f + tanhf(kGeluC0 * (x + kGeluC1 * x * x * x)));
+}
+
+__device__ __forceinline__ float gelu_grad(float x)
+{
+    const float t = tanhf(kGeluC0 * (x + kGeluC1 * x * x * x));
+    return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * kGeluC0 * (1.0f + 3.0f * kGeluC1 * x * x);
+}
+
+__device__ __forceinline__ int wrapi(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+__device__ __forceinline__ bf16x8 load8(const bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
+
+// ---- convolution forward / backward-data --------------------------------------------
+
+enum ConvMode { kOutBf16 = 0, kOutBf16Gelu = 1, kOutF32 = 2 };
+
+// out[p][co] = bias[co] + sum_{tap, ci} in[p + shift(tap)][ci] * w[tap][co][ci]
+// (flip: shift negated, used for the data gradient with the [tap][cin][cout] layout).
+template <int MB, int NB, int MODE>
+__global__ __launch_bounds__(256) void conv_kernel(const bf16* __restrict__ in, int cin, const bf16* __restrict__ w,
+                                                   const float* __restrict__ bias, int taps, int flip, int H, int W,
+                                                   void* __restrict__ out, bf16* __restrict__ pre)
+{
+    constexpr int COUT = 16 * NB;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int q = lane >> 4, l16 = lane & 15;
+    const int pix0 = (blockIdx.x * 4 + wave) * (16 * MB);
+    int pn[MB], py[MB], px[MB];
+#pragma unroll
+    for (int m = 0; m < MB; ++m) {
+        const int p = pix0 + 16 * m + l16;
+        px[m] = p % W;
+        const int t = p / W;
+        py[m] = t % H;
+        pn[m] = t / H;
+    }
+    f32x4 acc[MB][NB];
+#pragma unroll
+    for (int m = 0; m < MB; ++m)
+#pragma unroll
+        for (int n = 0; n < NB; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int K = taps * cin;
+    const bf16x8 zero = {};
+    for (int kb = 0; kb < K; kb += 32) {
+        const int k0 = kb + 8 * q;
+        bf16x8 a[MB], b[NB];
+        if (k0 < K) {
+            const int tap = k0 / cin, ci = k0 - tap * cin;
+            int dy = 0, dx = 0;
+            if (taps == 9) {
+                dy = tap / 3 - 1;
+                dx = tap % 3 - 1;
+                if (flip) {
+                    dy = -dy;
+                    dx = -dx;
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < MB; ++m) {
+                const int yy = wrapi(py[m] + dy, H), xx = wrapi(px[m] + dx, W);
+                a[m] = load8(in + (((size_t)pn[m] * H + yy) * W + xx) * cin + ci);
+            }
+#pragma unroll
+            for (int n = 0; n < NB; ++n) b[n] = load8(w + ((size_t)tap * COUT + 16 * n + l16) * cin + ci);
+        } else {
+#pragma unroll
+            for (int m = 0; m < MB; ++m) a[m] = zero;
+#pragma unroll
+            for (int n = 0; n < NB; ++n) b[n] = zero;
+        }
+#pragma unroll
+        for (int m = 0; m < MB; ++m)
+#pragma unroll
+            for (int n = 0; n < NB; ++n) acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b[n], acc[m][n], 0, 0, 0);
+    }
+#pragma unroll
+    for (int m = 0; m < MB; ++m)
+#pragma unroll
+        for (int n = 0; n < NB; ++n) {
+            const int co = 16 * n + l16;
+            const float bco = MODE == kOutF32 ? 0.f : bias[co];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const size_t idx = (size_t)(pix0 + 16 * m + 4 * q + r) * COUT + co;
+                const float v = acc[m][n][r] + bco;
+                if constexpr (MODE == kOutF32) {
+                    static_cast<float*>(out)[idx] = v;
+                } else if constexpr (MODE == kOutBf16Gelu) {
+                    pre[idx] = f2bf(v);
+                    static_cast<bf16*>(out)[idx] = f2bf(gelu(v));
+                } else {
+                    static_cast<bf16*>(out)[idx] = f2bf(v);
+                }
+            }
+        }
+}
+
+// ---- weight gradient ----------------------------------------------------------------
+// part[chunk][tap][co][ci] = sum_{p in chunk} dpre[p][co] * in[p + shift(tap)][ci].
+// 32-pixel K-steps are staged transposed in LDS so each lane reads its 8 pixels as
+// one 16-byte vector.
+template <int NBO, int NBI>
+__global__ __launch_bounds__(256) void wgrad_kernel(const bf16* __restrict__ dpre, const bf16* __restrict__ in,
+                                                    int taps, int H, int W, int chunk, float* __restrict__ part)
+{
+    constexpr int COUT = 16 * NBO, CIN = 16 * NBI, LD = 40;  // 32 pixels + 8 pad
+    constexpr int TILES = NBO * NBI, TPW = (TILES + 3) / 4;
+    __shared__ __align__(16) bf16 sA[COUT * LD];
+    __shared__ __align__(16) bf16 sB[CIN * LD];
+    const int c = blockIdx.x, tap = blockIdx.y;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int q = lane >> 4, l16 = lane & 15;
+    int dy = 0, dx = 0;
+    if (taps == 9) {
+        dy = tap / 3 - 1;
+        dx = tap % 3 - 1;
+    }
+    f32x4 acc[TPW];
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int pbeg = c * chunk;
+    for (int p0 = pbeg; p0 < pbeg + chunk; p0 += 32) {
+        for (int e = threadIdx.x; e < 32 * COUT; e += 256) {
+            const int pl = e / COUT, ch = e - pl * COUT;
+            sA[ch * LD + pl] = dpre[(size_t)(p0 + pl) * COUT + ch];
+        }
+        for (int e = threadIdx.x; e < 32 * CIN; e += 256) {
+            const int pl = e / CIN, ch = e - pl * CIN;
+            const int p = p0 + pl;
+            const int x = p % W, t = p / W, y = t % H, n = t / H;
+            sB[ch * LD + pl] = in[(((size_t)n * H + wrapi(y + dy, H)) * W + wrapi(x + dx, W)) * CIN + ch];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) {
+            const int tile = wave + 4 * i;
+            if (tile < TILES) {
+                const int ib = tile / NBI, jb = tile - ib * NBI;
+                const bf16x8 a = load8(sA + (16 * ib + l16) * LD + 8 * q);
+                const bf16x8 b = load8(sB + (16 * jb + l16) * LD + 8 * q);
+                acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    float* dst = part + ((size_t)c * taps + tap) * COUT * CIN;
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) {
+        const int tile = wave + 4 * i;
+        if (tile < TILES) {
+            const int ib = tile / NBI, jb = tile - ib * NBI;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) dst[(16 * ib + 4 * q + r) * CIN + 16 * jb + l16] = acc[i][r];
+        }
+    }
+}
+
+// Bias gradient partials: part[chunk][co] = sum_{p in chunk} dpre[p][co].
+__global__ void bgrad_kernel(const bf16* __restrict__ dpre, int cout, int chunk, float* __restrict__ part)
+{
+    const int c = blockIdx.x;
+    for (int co = threadIdx.x; co < cout; co += blockDim.x) {
+        float s = 0.f;
+        for (int p = c * chunk; p < (c + 1) * chunk; ++p) s += bf2f(dpre[(size_t)p * cout + co]);
+        part[(size_t)c * cout + co] = s;
+    }
+}
+
+// out[i] = sum_c part[c * len + i] (fixed order, deterministic).
+__global__ void reduce_chunks_kernel(const float* __restrict__ part, int nchunks, int len, float* __restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= len) return;
+    float s = 0.f;
+    for (int c = 0; c < nchunks; ++c) s += part[(size_t)c * len + i];
+    out[i] = s;
+}
+
+// ---- elementwise and resampling ------------------------------------------------------
+
+__global__ void input_kernel(const float* __restrict__ phi, const float* __restrict__ params, int hw, int npix,
+                             bf16* __restrict__ x0)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    const int n = p / hw;
+    bf16* dst = x0 + (size_t)p * kCin0;
+    dst[0] = f2bf(phi[p]);
+    dst[1] = f2bf(params[3 * n + 0] / kParamScale[0]);
+    dst[2] = f2bf(params[3 * n + 1] / kParamScale[1]);
+    dst[3] = f2bf(params[3 * n + 2] / kParamScale[2]);
+    for (int c = 4; c < kCin0; ++c) dst[c] = 0;
+}
+
+// out (N, H/2, W/2, C) = mean of 2x2 blocks of in (N, H, W, C).
+__global__ void avgpool_kernel(const bf16* __restrict__ in, int H, int W, int C, int nout, bf16* __restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nout) return;
+    const int c = i % C, t = i / C, Wo = W / 2, Ho = H / 2;
+    const int x = t % Wo, t2 = t / Wo, y = t2 % Ho, n = t2 / Ho;
+    const size_t base = (((size_t)n * H + 2 * y) * W + 2 * x) * C + c;
+    const float s = bf2f(in[base]) + bf2f(in[base + C]) + bf2f(in[base + (size_t)W * C]) + bf2f(in[base + (size_t)W * C + C]);
+    out[i] = f2bf(0.25f * s);
+}
+
+// s (N, H, W, C) = up2(u) + skip, u of size (N, H/2, W/2, C).
+__global__ void upadd_kernel(const bf16* __restrict__ u, const bf16* __restrict__ skip, int H, int W, int C, int n,
+                             bf16* __restrict__ s)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = i % C, t = i / C, x = t % W, t2 = t / W, y = t2 % H, b = t2 / H;
+    const float v = bf2f(u[(((size_t)b * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c]) + bf2f(skip[i]);
+    s[i] = f2bf(v);
+}
+
+// Up-sampling backward: out (N, H/2, W/2, C) bf16 = sum of the 2x2 block of g.
+__global__ void sumpool_kernel(const float* __restrict__ g, int H, int W, int C, int nout, bf16* __restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nout) return;
+    const int c = i % C, t = i / C, Wo = W / 2, Ho = H / 2;
+    const int x = t % Wo, t2 = t / Wo, y = t2 % Ho, b = t2 / Ho;
+    const size_t base = (((size_t)b * H + 2 * y) * W + 2 * x) * C + c;
+    out[i] = f2bf(g[base] + g[base + C] + g[base + (size_t)W * C] + g[base + (size_t)W * C + C]);
+}
+
+// dpre = bf16((g + pool_grad) * gelu'(pre)); pool_grad(i) = gc[coarse(i)] / 4 when gc is given.
+__global__ void gelu_bwd_kernel(const float* __restrict__ g, const float* __restrict__ gc, const bf16* __restrict__ pre,
+                                int H, int W, int C, int n, bf16* __restrict__ dpre)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float v = g[i];
+    if (gc) {
+        const int c = i % C, t = i / C, x = t % W, t2 = t / W, y = t2 % H, b = t2 / H;
+        v += 0.25f * gc[(((size_t)b * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c];
+    }
+    dpre[i] = f2bf(v * gelu_grad(bf2f(pre[i])));
+}
+
+// ---- head (32 -> 2 channels, FP32 weights and output) ----------------------------------
+
+__global__ void head_fwd_kernel(const bf16* __restrict__ d1, const float* __restrict__ wh, const float* __restrict__ bh,
+                                int H, int W, int npix, float* __restrict__ J)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    const int x = p % W, t = p / W, y = t % H, n = t / H;
+    float j0 = bh[0], j1 = bh[1];
+    for (int tap = 0; tap < 9; ++tap) {
+        const int yy = wrapi(y + tap / 3 - 1, H), xx = wrapi(x + tap % 3 - 1, W);
+        const bf16* src = d1 + (((size_t)n * H + yy) * W + xx) * 32;
+        const float* w0 = wh + tap * 64;
+#pragma unroll
+        for (int ci = 0; ci < 32; ++ci) {
+            const float v = bf2f(src[ci]);
+            j0 = fmaf(v, w0[ci], j0);
+            j1 = fmaf(v, w0[32 + ci], j1);
+        }
+    }
+    J[2 * (size_t)p] = j0;
+    J[2 * (size_t)p + 1] = j1;
+}
+
+// dd1[p][ci] = sum_{tap, c} dJ[p - shift(tap)][c] * wh[tap][c][ci]
+__global__ void head_bwd_data_kernel(const float* __restrict__ dJ, const float* __restrict__ wh, int H, int W, int npix,
+                                     float* __restrict__ dd1)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    const int x = p % W, t = p / W, y = t % H, n = t / H;
+    float acc[32];
+#pragma unroll
+    for (int ci = 0; ci < 32; ++ci) acc[ci] = 0.f;
+    for (int tap = 0; tap < 9; ++tap) {
+        const int yy = wrapi(y - (tap / 3 - 1), H), xx = wrapi(x - (tap % 3 - 1), W);
+        const size_t q = ((size_t)n * H + yy) * W + xx;
+        const float g0 = dJ[2 * q], g1 = dJ[2 * q + 1];
+        const float* w0 = wh + tap * 64;
+#pragma unroll
+        for (int ci = 0; ci < 32; ++ci) acc[ci] = fmaf(g0, w0[ci], fmaf(g1, w0[32 + ci], acc[ci]));
+    }
+#pragma unroll
+    for (int ci = 0; ci < 32; ++ci) dd1[(size_t)p * 32 + ci] = acc[ci];
+}
+
+// part[chunk][tap*64 + c*32 + ci] = sum_p dJ[p][c] d1[p + shift(tap)][ci]; part[chunk][576 + c] = sum_p dJ[p][c]
+__global__ __launch_bounds__(320) void head_bwd_weight_kernel(const float* __restrict__ dJ, const bf16* __restrict__ d1,
+                                                              int H, int W, int chunk, float* __restrict__ part)
+{
+    const int c = blockIdx.x, t = threadIdx.x;
+    if (t >= 290) return;
+    float s0 = 0.f, s1 = 0.f;
+    if (t < 288) {
+        const int tap = t / 32, ci = t % 32;
+        for (int p = c * chunk; p < (c + 1) * chunk; ++p) {
+            const int x = p % W, tt = p / W, y = tt % H, n = tt / H;
+            const float v = bf2f(d1[(((size_t)n * H + wrapi(y + tap / 3 - 1, H)) * W + wrapi(x + tap % 3 - 1, W)) * 32 + ci]);
+            s0 = fmaf(dJ[2 * (size_t)p], v, s0);
+            s1 = fmaf(dJ[2 * (size_t)p + 1], v, s1);
+        }
+        float* dst = part + (size_t)c * 578;
+        dst[tap * 64 + ci] = s0;
+        dst[tap * 64 + 32 + ci] = s1;
+    } else {
+        const int comp = t - 288;
+        for (int p = c * chunk; p < (c + 1) * chunk; ++p) s0 += dJ[2 * (size_t)p + comp];
+        part[(size_t)c * 578 + 576 + comp] = s0;
+    }
+}
+
+// ---- flux-form update, loss ------------------------------------------------------------
+
+// out = phi - div J with the backward-difference divergence (exactly conservative).
+__global__ void flux_update_kernel(const float* __restrict__ phi, const float* __restrict__ J, int H, int W, int npix,
+                                   float* __restrict__ out)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    const int x = p % W, t = p / W, y = t % H, n = t / H;
+    const size_t up = ((size_t)n * H + wrapi(y - 1, H)) * W + x;
+    const size_t left = ((size_t)n * H + y) * W + wrapi(x - 1, W);
+    const float div = (J[2 * (size_t)p] - J[2 * up]) + (J[2 * (size_t)p + 1] - J[2 * left + 1]);
+    out[p] = phi[p] - div;
+}
+
+// g = 2 (pred - target) / (norm_n N); part[block] = sum of (pred - target)^2 / norm_n.
+__global__ __launch_bounds__(256) void loss_kernel(const float* __restrict__ pred, const float* __restrict__ target,
+                                                   const float* __restrict__ norm, int hw, int npix, int batch,
+                                                   float* __restrict__ g, float* __restrict__ part)
+{
+    __shared__ float s[256];
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    float v = 0.f;
+    if (p < npix) {
+        const int n = p / hw;
+        const float r = pred[p] - target[p];
+        v = r * r / norm[n];
+        g[p] = 2.f * r / (norm[n] * batch);
+    }
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+// dJ = -div^T g: dJy(i,j) = g(i+1,j) - g(i,j), dJx(i,j) = g(i,j+1) - g(i,j).
+__global__ void dj_kernel(const float* __restrict__ g, int H, int W, int npix, float* __restrict__ dJ)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    const int x = p % W, t = p / W, y = t % H, n = t / H;
+    const float gp = g[p];
+    dJ[2 * (size_t)p] = g[((size_t)n * H + wrapi(y + 1, H)) * W + x] - gp;
+    dJ[2 * (size_t)p + 1] = g[((size_t)n * H + y) * W + wrapi(x + 1, W)] - gp;
+}
+
+// ---- optimiser -------------------------------------------------------------------------
+
+__global__ void sumsq_kernel(const float* __restrict__ g, int n, float* __restrict__ part)
+{
+    __shared__ float s[256];
+    float v = 0.f;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) v += g[i] * g[i];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+// Adam with bias correction; the gradient is first scaled by min(1, clip / ||g||).
+__global__ void adam_kernel(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                            float* __restrict__ v, int n, const float* __restrict__ sumsq_part, int nparts, float lr,
+                            float b1, float b2, float eps, float bc1, float bc2, float clip)
+{
+    __shared__ float scale;
+    if (threadIdx.x == 0) {
+        float s = 0.f;
+        for (int i = 0; i < nparts; ++i) s += sumsq_part[i];
+        const float norm = sqrtf(s);
+        scale = (clip > 0.f && norm > clip) ? clip / norm : 1.f;
+    }
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i] * scale;
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+}
+
+// BF16 forward [tap][co][ci] and backward-data [tap][ci][co] copies of a layer's weights.
+__global__ void pack_kernel(const float* __restrict__ master, int taps, int cout, int cin, bf16* __restrict__ wf,
+                            bf16* __restrict__ wb)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= taps * cout * cin) return;
+    const int ci = i % cin, t = i / cin, co = t % cout, tap = t / cout;
+    const bf16 v = f2bf(master[i]);
+    wf[i] = v;
+    wb[((size_t)tap * cin + ci) * cout + co] = v;
+}
+
+__global__ void gather_kernel(const float* __restrict__ src, const int* __restrict__ idx, int per, int count,
+                              float* __restrict__ dst)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= per * count) return;
+    dst[i] = src[(size_t)idx[i / per] * per + i % per];
+}
+
+__global__ void f64_to_f32_kernel(const double* __restrict__ a, int n, float* __restrict__ b)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
+}  // namespace unet
+}  // namespace amb
+
+using namespace amb;
+using namespace amb::unet;
+
+namespace {
+
+struct LayerDef {
+    int cin, cout, taps;
+    bool act;
+};
+
+// c1 .. c7 on matrix cores, then the FP32 head.
+const LayerDef kLayers[8] = {{kCin0, 32, 9, true}, {32, 64, 9, true},  {64, 128, 9, true}, {128, 64, 1, false},
+                             {64, 64, 9, true},    {64, 32, 1, false}, {32, 32, 9, true},  {32, 2, 9, false}};
+constexpr int kEw = 256;  // threads of elementwise kernels
+
+unsigned blocks_for(size_t n, int t = kEw) { return (unsigned)((n + t - 1) / t); }
+
+// Pixels per weight-gradient partial: the largest of 4096 .. 256 dividing npix
+// (npix is always a multiple of 256 since grid sides are multiples of 64).
+int chunk_for(int npix)
+{
+    for (int c : {4096, 2048, 1024, 512, 256})
+        if (npix % c == 0) return c;
+    return 0;
+}
+
+}  // namespace
+
+struct amb_surrogate {
+    int H = 0, W = 0, max_batch = 0;
+    hipStream_t stream = nullptr;
+    size_t w_off[8], b_off[8], nparams = 0;
+    float* master = nullptr;  // FP32 weights and biases (flat)
+    float* grad = nullptr;
+    float* adam_m = nullptr;
+    float* adam_v = nullptr;
+    int64_t adam_t = 0;
+    bf16* wf[7] = {};
+    bf16* wb[7] = {};
+    // activations (bf16 NHWC) and gradients
+    bf16 *x0 = nullptr, *e1 = nullptr, *e1p = nullptr, *p1 = nullptr, *e2 = nullptr, *e2p = nullptr, *p2 = nullptr;
+    bf16 *bt = nullptr, *btp = nullptr, *u2 = nullptr, *s2 = nullptr, *d2 = nullptr, *d2p = nullptr, *u1 = nullptr;
+    bf16 *s1 = nullptr, *d1 = nullptr, *d1p = nullptr, *dpre = nullptr;
+    float *J = nullptr, *phi = nullptr, *target = nullptr, *params = nullptr, *norm = nullptr, *pred = nullptr;
+    float *g = nullptr, *dJ = nullptr, *dd1 = nullptr, *ds1 = nullptr, *dd2 = nullptr, *ds2 = nullptr, *dbt = nullptr;
+    float *dp2 = nullptr, *dp1 = nullptr, *part = nullptr, *loss_part = nullptr, *sumsq_part = nullptr;
+    size_t part_len = 0;
+    int* d_idx = nullptr;
+    // dataset (device): phi_in, phi_out [S][H][W], params [S][3], norms [S]
+    float *ds_in = nullptr, *ds_out = nullptr, *ds_par = nullptr, *ds_norm = nullptr;
+    int64_t ds_count = 0;
+    std::mt19937_64 rng;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipGraphExec_t step_graph = nullptr;  // one surrogate step on a field, batch 1
+    void* graph_field = nullptr;
+    void* graph_out = nullptr;
+};
+
+namespace {
+
+std::vector<void*> all_buffers(amb_surrogate* s)
+{
+    std::vector<void*> v = {s->master, s->grad, s->adam_m, s->adam_v, s->x0, s->e1, s->e1p, s->p1, s->e2, s->e2p,
+                            s->p2, s->bt, s->btp, s->u2, s->s2, s->d2, s->d2p, s->u1, s->s1, s->d1, s->d1p, s->dpre,
+                            s->J, s->phi, s->target, s->params, s->norm, s->pred, s->g, s->dJ, s->dd1, s->ds1,
+                            s->dd2, s->ds2, s->dbt, s->dp2, s->dp1, s->part, s->loss_part, s->sumsq_part, s->d_idx,
+                            s->ds_in, s->ds_out, s->ds_par, s->ds_norm};
+    for (int l = 0; l < 7; ++l) {
+        v.push_back(s->wf[l]);
+        v.push_back(s->wb[l]);
+    }
+    return v;
+}
+
+void release(amb_surrogate* s)
+{
+    if (!s) return;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->step_graph) (void)hipGraphExecDestroy(s->step_graph);
+    for (void* p : all_buffers(s))
+        if (p) (void)hipFree(p);
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+template <int MB, int NB, int MODE>
+hipError_t conv_launch(const bf16* in, int cin, const bf16* w, const float* bias, int taps, int flip, int npix, int H,
+                       int W, void* out, bf16* pre, hipStream_t st)
+{
+    hipLaunchKernelGGL((conv_kernel<MB, NB, MODE>), dim3(npix / (64 * MB)), dim3(256), 0, st, in, cin, w, bias, taps,
+                       flip, H, W, out, pre);
+    return hipGetLastError();
+}
+
+// Dispatch on output channels (16 NB) with MB chosen to keep 64 accumulator registers.
+template <int MODE>
+hipError_t conv(const bf16* in, int cin, const bf16* w, const float* bias, int cout, int taps, int flip, int npix,
+                int H, int W, void* out, bf16* pre, hipStream_t st)
+{
+    switch (cout) {
+    case 16: return conv_launch<4, 1, MODE>(in, cin, w, bias, taps, flip, npix, H, W, out, pre, st);
+    case 32: return conv_launch<4, 2, MODE>(in, cin, w, bias, taps, flip, npix, H, W, out, pre, st);
+    case 64: return conv_launch<4, 4, MODE>(in, cin, w, bias, taps, flip, npix, H, W, out, pre, st);
+    case 128: return conv_launch<2, 8, MODE>(in, cin, w, bias, taps, flip, npix, H, W, out, pre, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int NBO, int NBI>
+hipError_t wgrad_launch(const bf16* dpre, const bf16* in, int taps, int H, int W, int npix, float* part, hipStream_t st)
+{
+    const int chunk = chunk_for(npix);
+    hipLaunchKernelGGL((wgrad_kernel<NBO, NBI>), dim3(npix / chunk, taps), dim3(256), 0, st, dpre, in, taps, H, W,
+                       chunk, part);
+    return hipGetLastError();
+}
+
+hipError_t wgrad(const bf16* dpre, int cout, const bf16* in, int cin, int taps, int H, int W, int npix, float* part,
+                 hipStream_t st)
+{
+#define AMB_WG(co, ci) \
+    if (cout == 16 * co && cin == 16 * ci) return wgrad_launch<co, ci>(dpre, in, taps, H, W, npix, part, st);
+    AMB_WG(2, 1)  // c1: 32 <- 16
+    AMB_WG(4, 2)  // c2: 64 <- 32
+    AMB_WG(8, 4)  // c3: 128 <- 64
+    AMB_WG(4, 8)  // c4: 64 <- 128
+    AMB_WG(4, 4)  // c5: 64 <- 64
+    AMB_WG(2, 4)  // c6: 32 <- 64
+    AMB_WG(2, 2)  // c7: 32 <- 32
+#undef AMB_WG
+    return hipErrorInvalidValue;
+}
+
+// Weight and bias gradient of conv layer l into s->grad.
+int layer_grads(amb_surrogate* s, int l, const bf16* dpre, const bf16* in, int H, int W, int npix)
+{
+    const LayerDef& L = kLayers[l];
+    const int chunk = chunk_for(npix), nch = npix / chunk;
+    const int len = L.taps * L.cout * L.cin;
+    hipError_t e = wgrad(dpre, L.cout, in, L.cin, L.taps, H, W, npix, s->part, s->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(len)), dim3(kEw), 0, s->stream, s->part, nch, len,
+                           s->grad + s->w_off[l]);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(bgrad_kernel, dim3(nch), dim3(128), 0, s->stream, dpre, L.cout, chunk, s->part);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(L.cout)), dim3(kEw), 0, s->stream, s->part, nch,
+                           L.cout, s->grad + s->b_off[l]);
+        e = hipGetLastError();
+    }
+    return e == hipSuccess ? AMB_OK : fail_hip(e, "surrogate weight gradient");
+}
+
+int pack_weights(amb_surrogate* s)
+{
+    for (int l = 0; l < 7; ++l) {
+        const LayerDef& L = kLayers[l];
+        const int n = L.taps * L.cout * L.cin;
+        hipLaunchKernelGGL(pack_kernel, dim3(blocks_for(n)), dim3(kEw), 0, s->stream, s->master + s->w_off[l], L.taps,
+                           L.cout, L.cin, s->wf[l], s->wb[l]);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return fail_hip(e, "weight pack");
+    }
+    return AMB_OK;
+}
+
+#define SUR_CHECK(expr)                                   \
+    do {                                                  \
+        const hipError_t e_ = (expr);                     \
+        if (e_ != hipSuccess) return fail_hip(e_, #expr); \
+    } while (0)
+
+// Forward pass on s->phi / s->params (batch n) into s->J; with `train` the
+// pre-activations are kept for the backward pass.
+int forward(amb_surrogate* s, int n, const float* phi, const float* params, float* J)
+{
+    const int H = s->H, W = s->W, H2 = H / 2, W2 = W / 2, H4 = H / 4, W4 = W / 4;
+    const int P1 = n * H * W, P2 = n * H2 * W2, P3 = n * H4 * W4;
+    hipStream_t st = s->stream;
+    const float* M = s->master;
+    hipLaunchKernelGGL(input_kernel, dim3(blocks_for(P1)), dim3(kEw), 0, st, phi, params, H * W, P1, s->x0);
+    SUR_CHECK(hipGetLastError());
+    SUR_CHECK(conv<kOutBf16Gelu>(s->x0, kCin0, s->wf[0], M + s->b_off[0], 32, 9, 0, P1, H, W, s->e1, s->e1p, st));
+    hipLaunchKernelGGL(avgpool_kernel, dim3(blocks_for((size_t)P2 * 32)), dim3(kEw), 0, st, s->e1, H, W, 32, P2 * 32, s->p1);
+    SUR_CHECK(hipGetLastError());
+    SUR_CHECK(conv<kOutBf16Gelu>(s->p1, 32, s->wf[1], M + s->b_off[1], 64, 9, 0, P2, H2, W2, s->e2, s->e2p, st));
+    hipLaunchKernelGGL(avgpool_kernel, dim3(blocks_for((size_t)P3 * 64)), dim3(kEw), 0, st, s->e2, H2, W2, 64, P3 * 64, s->p2);
+    SUR_CHECK(hipGetLastError());
+    SUR_CHECK(conv<kOutBf16Gelu>(s->p2, 64, s->wf[2], M + s->b_off[2], 128, 9, 0, P3, H4, W4, s->bt, s->btp, st));
+    SUR_CHECK(conv<kOutBf16>(s->bt, 128, s->wf[3], M + s->b_off[3], 64, 1, 0, P3, H4, W4, s->u2, nullptr, st));
+    hipLaunchKernelGGL(upadd_kernel, dim3(blocks_for((size_t)P2 * 64)), dim3(kEw), 0, st, s->u2, s->e2, H2, W2, 64, P2 * 64, s->s2);
+    SUR_CHECK(hipGetLastError());
+    SUR_CHECK(conv<kOutBf16Gelu>(s->s2, 64, s->wf[4], M + s->b_off[4], 64, 9, 0, P2, H2, W2, s->d2, s->d2p, st));
+    SUR_CHECK(conv<kOutBf16>(s->d2, 64, s->wf[5], M + s->b_off[5], 32, 1, 0, P2, H2, W2, s->u1, nullptr, st));
+    hipLaunchKernelGGL(upadd_kernel, dim3(blocks_for((size_t)P1 * 32)), dim3(kEw), 0, st, s->u1, s->e1, H, W, 32, P1 * 32, s->s1);
+    SUR_CHECK(hipGetLastError());
+    SUR_CHECK(conv<kOutBf16Gelu>(s->s1, 32, s->wf[6], M + s->b_off[6], 32, 9, 0, P1, H, W, s->d1, s->d1p, st));
+    hipLaunchKernelGGL(head_fwd_kernel, dim3(blocks_for(P1)), dim3(kEw), 0, st, s->d1, M + s->w_off[7], M + s->b_off[7],
+                       H, W, P1, J);
+    SUR_CHECK(hipGetLastError());
+    return AMB_OK;
+}
+
+// Loss and full gradient into s->grad for the batch in s->phi, s->params, s->target, s->norm.
+int loss_and_grad(amb_surrogate* s, int n, double* loss_out)
+{
+    const int H = s->H, W = s->W, H2 = H / 2, W2 = W / 2, H4 = H / 4, W4 = W / 4;
+    const int P1 = n * H * W, P2 = n * H2 * W2, P3 = n * H4 * W4;
+    hipStream_t st = s->stream;
+    const float* M = s->master;
+    int rc = forward(s, n, s->phi, s->params, s->J);
+    if (rc != AMB_OK) return rc;
+    hipLaunchKernelGGL(flux_update_kernel, dim3(blocks_for(P1)), dim3(kEw), 0, st, s->phi, s->J, H, W, P1, s->pred);
+    const unsigned lb = blocks_for(P1);
+    hipLaunchKernelGGL(loss_kernel, dim3(lb), dim3(256), 0, st, s->pred, s->target, s->norm, H * W, P1, n, s->g,
+                       s->loss_part);
+    hipLaunchKernelGGL(dj_kernel, dim3(blocks_for(P1)), dim3(kEw), 0, st, s->g, H, W, P1, s->dJ);
+    SUR_CHECK(hipGetLastError());
+    // head
+    hipLaunchKernelGGL(head_bwd_data_kernel, dim3(blocks_for(P1)), dim3(kEw), 0, st, s->dJ, M + s->w_off[7], H, W, P1,
+                       s->dd1);
+    const int hchunk = chunk_for(P1), hch = P1 / hchunk;
+    hipLaunchKernelGGL(head_bwd_weight_kernel, dim3(hch), dim3(320), 0, st, s->dJ, s->d1, H, W, hchunk, s->part);
+    hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(578)), dim3(kEw), 0, st, s->part, hch, 578,
+                       s->grad + s->w_off[7]);
+    SUR_CHECK(hipGetLastError());
+    // c7 (in s1), then s1 = up(u1) + e1
+    hipLaunchKernelGGL(gelu_bwd_kernel, dim3(blocks_for((size_t)P1 * 32)), dim3(kEw), 0, st, s->dd1, nullptr, s->d1p,
+                       H, W, 32, P1 * 32, s->dpre);
+    if ((rc = layer_grads(s, 6, s->dpre, s->s1, H, W, P1)) != AMB_OK) return rc;
+    SUR_CHECK(conv<kOutF32>(s->dpre, 32, s->wb[6], nullptr, 32, 9, 1, P1, H, W, s->ds1, nullptr, st));
+    hipLaunchKernelGGL(sumpool_kernel, dim3(blocks_for((size_t)P2 * 32)), dim3(kEw), 0, st, s->ds1, H, W, 32, P2 * 32,
+                       s->dpre);
+    // c6 (1x1, in d2)
+    if ((rc = layer_grads(s, 5, s->dpre, s->d2, H2, W2, P2)) != AMB_OK) return rc;
+    SUR_CHECK(conv<kOutF32>(s->dpre, 32, s->wb[5], nullptr, 64, 1, 1, P2, H2, W2, s->dd2, nullptr, st));
+    // c5 (in s2), then s2 = up(u2) + e2
+    hipLaunchKernelGGL(gelu_bwd_kernel, dim3(blocks_for((size_t)P2 * 64)), dim3(kEw), 0, st, s->dd2, nullptr, s->d2p,
+                       H2, W2, 64, P2 * 64, s->dpre);
+    if ((rc = layer_grads(s, 4, s->dpre, s->s2, H2, W2, P2)) != AMB_OK) return rc;
+    SUR_CHECK(conv<kOutF32>(s->dpre, 64, s->wb[4], nullptr, 64, 9, 1, P2, H2, W2, s->ds2, nullptr, st));
+    hipLaunchKernelGGL(sumpool_kernel, dim3(blocks_for((size_t)P3 * 64)), dim3(kEw), 0, st, s->ds2, H2, W2, 64, P3 * 64,
+                       s->dpre);
+    // c4 (1x1, in bt)
+    if ((rc = layer_grads(s, 3, s->dpre, s->bt, H4, W4, P3)) != AMB_OK) return rc;
+    SUR_CHECK(conv<kOutF32>(s->dpre, 64, s->wb[3], nullptr, 128, 1, 1, P3, H4, W4, s->dbt, nullptr, st));
+    // c3 (in p2)
+    hipLaunchKernelGGL(gelu_bwd_kernel, dim3(blocks_for((size_t)P3 * 128)), dim3(kEw), 0, st, s->dbt, nullptr, s->btp,
+                       H4, W4, 128, P3 * 128, s->dpre);
+    if ((rc = layer_grads(s, 2, s->dpre, s->p2, H4, W4, P3)) != AMB_OK) return rc;
+    SUR_CHECK(conv<kOutF32>(s->dpre, 128, s->wb[2], nullptr, 64, 9, 1, P3, H4, W4, s->dp2, nullptr, st));
+    // c2 (in p1): de2 = ds2 + pool_grad(dp2)
+    hipLaunchKernelGGL(gelu_bwd_kernel, dim3(blocks_for((size_t)P2 * 64)), dim3(kEw), 0, st, s->ds2, s->dp2, s->e2p,
+                       H2, W2, 64, P2 * 64, s->dpre);
+    if ((rc = layer_grads(s, 1, s->dpre, s->p1, H2, W2, P2)) != AMB_OK) return rc;
+    SUR_CHECK(conv<kOutF32>(s->dpre, 64, s->wb[1], nullptr, 32, 9, 1, P2, H2, W2, s->dp1, nullptr, st));
+    // c1 (in x0): de1 = ds1 + pool_grad(dp1)
+    hipLaunchKernelGGL(gelu_bwd_kernel, dim3(blocks_for((size_t)P1 * 32)), dim3(kEw), 0, st, s->ds1, s->dp1, s->e1p,
+                       H, W, 32, P1 * 32, s->dpre);
+    if ((rc = layer_grads(s, 0, s->dpre, s->x0, H, W, P1)) != AMB_OK) return rc;
+    SUR_CHECK(hipGetLastError());
+    if (loss_out) {
+        std::vector<float> lp(lb);
+        SUR_CHECK(hipMemcpyAsync(lp.data(), s->loss_part, lb * sizeof(float), hipMemcpyDeviceToHost, st));
+        SUR_CHECK(hipStreamSynchronize(st));
+        double sum = 0;
+        for (float v : lp) sum += v;
+        *loss_out = sum / n;
+    }
+    return AMB_OK;
+}
+
+int adam_step(amb_surrogate* s, float lr, float clip)
+{
+    const int n = (int)s->nparams;
+    const int nparts = 64;
+    hipLaunchKernelGGL(sumsq_kernel, dim3(nparts), dim3(256), 0, s->stream, s->grad, n, s->sumsq_part);
+    s->adam_t += 1;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float bc1 = 1.f - std::pow(b1, (float)s->adam_t), bc2 = 1.f - std::pow(b2, (float)s->adam_t);
+    hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(n)), dim3(kEw), 0, s->stream, s->master, s->grad, s->adam_m,
+                       s->adam_v, n, s->sumsq_part, nparts, lr, b1, b2, 1e-8f, bc1, bc2, clip);
+    SUR_CHECK(hipGetLastError());
+    return pack_weights(s);
+}
+
+int check_dims(int64_t ny, int64_t nx)
+{
+    if (ny < 64 || nx < 64 || ny % 64 || nx % 64 || ny > 4096 || nx > 4096)
+        return fail(AMB_E_ARG, "surrogate grids must be multiples of 64 between 64 and 4096");
+    return AMB_OK;
+}
+
+// Upload host doubles for a batch into phi/params/target/norm.
+int upload_batch(amb_surrogate* s, int n, const double* phi, const double* params, const double* target)
+{
+    const size_t hw = (size_t)s->H * s->W;
+    std::vector<float> f(hw * n), p(3 * (size_t)n), t, nm;
+    for (size_t i = 0; i < f.size(); ++i) f[i] = (float)phi[i];
+    for (size_t i = 0; i < p.size(); ++i) p[i] = (float)params[i];
+    SUR_CHECK(hipMemcpyAsync(s->phi, f.data(), f.size() * 4, hipMemcpyHostToDevice, s->stream));
+    SUR_CHECK(hipMemcpyAsync(s->params, p.data(), p.size() * 4, hipMemcpyHostToDevice, s->stream));
+    if (target) {
+        t.resize(hw * n);
+        nm.assign(n, 0.f);
+        for (int b = 0; b < n; ++b) {
+            double acc = 0;
+            for (size_t i = 0; i < hw; ++i) {
+                t[b * hw + i] = (float)target[b * hw + i];
+                const double d = target[b * hw + i] - phi[b * hw + i];
+                acc += d * d;
+            }
+            nm[b] = (float)std::max(acc, 1e-30);
+        }
+        SUR_CHECK(hipMemcpyAsync(s->target, t.data(), t.size() * 4, hipMemcpyHostToDevice, s->stream));
+        SUR_CHECK(hipMemcpyAsync(s->norm, nm.data(), nm.size() * 4, hipMemcpyHostToDevice, s->stream));
+    }
+    SUR_CHECK(hipStreamSynchronize(s->stream));
+    return AMB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int amb_surrogate_create(int64_t ny, int64_t nx, int max_batch, uint64_t seed, amb_surrogate** out)
+{
+    clear_error();
+    if (!out) return fail(AMB_E_ARG, "out must not be null");
+    *out = nullptr;
+    int rc = check_dims(ny, nx);
+    if (rc != AMB_OK) return rc;
+    if (max_batch < 1 || max_batch > 256) return fail(AMB_E_ARG, "max_batch must be in [1, 256]");
+    if ((rc = check_device()) != AMB_OK) return rc;
+    amb_surrogate* s = new (std::nothrow) amb_surrogate();
+    if (!s) return fail(AMB_E_ARG, "out of host memory");
+    s->H = (int)ny;
+    s->W = (int)nx;
+    s->max_batch = max_batch;
+    s->rng.seed(seed ^ 0x9E3779B97F4A7C15ull);
+    size_t off = 0;
+    for (int l = 0; l < 8; ++l) {
+        s->w_off[l] = off;
+        off += (size_t)kLayers[l].taps * kLayers[l].cout * kLayers[l].cin;
+        s->b_off[l] = off;
+        off += kLayers[l].cout;
+    }
+    s->nparams = off;
+    const size_t B = max_batch, P1 = B * ny * nx, P2 = P1 / 4, P3 = P1 / 16;
+    // Partial-sum buffer: the most chunks x entries any layer needs at any batch size.
+    s->part_len = 0;
+    for (int b = 1; b <= max_batch; ++b) {
+        const int lv[3] = {(int)(b * ny * nx), (int)(b * ny * nx / 4), (int)(b * ny * nx / 16)};
+        const int level_of[8] = {0, 1, 2, 2, 1, 1, 0, 0};
+        for (int l = 0; l < 8; ++l) {
+            const int npix = lv[level_of[l]];
+            const size_t len = l == 7 ? 578 : (size_t)kLayers[l].taps * kLayers[l].cout * kLayers[l].cin;
+            s->part_len = std::max(s->part_len, (size_t)(npix / chunk_for(npix)) * len);
+        }
+    }
+    hipError_t e = hipSuccess;
+    auto A = [&](auto** p, size_t bytes) {
+        if (e == hipSuccess) e = hipMalloc((void**)p, std::max<size_t>(bytes, 256));
+    };
+    A(&s->master, s->nparams * 4);
+    A(&s->grad, s->nparams * 4);
+    A(&s->adam_m, s->nparams * 4);
+    A(&s->adam_v, s->nparams * 4);
+    for (int l = 0; l < 7; ++l) {
+        const size_t n = (size_t)kLayers[l].taps * kLayers[l].cout * kLayers[l].cin;
+        A(&s->wf[l], n * 2);
+        A(&s->wb[l], n * 2);
+    }
+    A(&s->x0, P1 * kCin0 * 2);
+    A(&s->e1, P1 * 32 * 2);
+    A(&s->e1p, P1 * 32 * 2);
+    A(&s->p1, P2 * 32 * 2);
+    A(&s->e2, P2 * 64 * 2);
+    A(&s->e2p, P2 * 64 * 2);
+    A(&s->p2, P3 * 64 * 2);
+    A(&s->bt, P3 * 128 * 2);
+    A(&s->btp, P3 * 128 * 2);
+    A(&s->u2, P3 * 64 * 2);
+    A(&s->s2, P2 * 64 * 2);
+    A(&s->d2, P2 * 64 * 2);
+    A(&s->d2p, P2 * 64 * 2);
+    A(&s->u1, P2 * 32 * 2);
+    A(&s->s1, P1 * 32 * 2);
+    A(&s->d1, P1 * 32 * 2);
+    A(&s->d1p, P1 * 32 * 2);
+    A(&s->dpre, P1 * 32 * 2);
+    A(&s->J, P1 * 2 * 4);
+    A(&s->phi, P1 * 4);
+    A(&s->target, P1 * 4);
+    A(&s->params, B * 3 * 4);
+    A(&s->norm, B * 4);
+    A(&s->pred, P1 * 4);
+    A(&s->g, P1 * 4);
+    A(&s->dJ, P1 * 2 * 4);
+    A(&s->dd1, P1 * 32 * 4);
+    A(&s->ds1, P1 * 32 * 4);
+    A(&s->dd2, P2 * 64 * 4);
+    A(&s->ds2, P2 * 64 * 4);
+    A(&s->dbt, P3 * 128 * 4);
+    A(&s->dp2, P3 * 64 * 4);
+    A(&s->dp1, P2 * 32 * 4);
+    A(&s->part, s->part_len * 4);
+    A(&s->loss_part, (P1 / 256 + 1) * 4);
+    A(&s->sumsq_part, 64 * 4);
+    A(&s->d_idx, B * 4);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&s->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&s->ev1);
+    if (e != hipSuccess) {
+        rc = fail_hip(e, "surrogate allocation");
+        release(s);
+        return rc;
+    }
+    // He-uniform initialisation (zero on c1's padding channels), zero biases and moments.
+    std::vector<float> init(s->nparams, 0.f);
+    std::mt19937_64 g(seed);
+    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    for (int l = 0; l < 8; ++l) {
+        const LayerDef& L = kLayers[l];
+        const int real_cin = l == 0 ? 4 : L.cin;
+        double bound = std::sqrt(6.0 / (L.taps * real_cin));
+        if (l == 7) bound *= 0.1;
+        for (int t = 0; t < L.taps; ++t)
+            for (int co = 0; co < L.cout; ++co)
+                for (int ci = 0; ci < L.cin; ++ci)
+                    init[s->w_off[l] + ((size_t)t * L.cout + co) * L.cin + ci] =
+                        (l == 0 && ci >= 4) ? 0.f : (float)(bound * U(g));
+    }
+    std::vector<float> zeros(s->nparams, 0.f);
+    e = hipMemcpy(s->master, init.data(), s->nparams * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->adam_m, zeros.data(), s->nparams * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->adam_v, zeros.data(), s->nparams * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        rc = fail_hip(e, "surrogate initialisation");
+        release(s);
+        return rc;
+    }
+    if ((rc = pack_weights(s)) != AMB_OK || hipStreamSynchronize(s->stream) != hipSuccess) {
+        release(s);
+        return rc != AMB_OK ? rc : fail(AMB_E_HIP, "surrogate initialisation");
+    }
+    *out = s;
+    return AMB_OK;
+}
+
+int amb_surrogate_destroy(amb_surrogate* s)
+{
+    clear_error();
+    release(s);
+    return AMB_OK;
+}
+
+int amb_surrogate_param_count(const amb_surrogate* s, int64_t* n)
+{
+    if (!s || !n) return fail(AMB_E_ARG, "surrogate and n must not be null");
+    *n = (int64_t)s->nparams;
+    return AMB_OK;
+}
+
+int amb_surrogate_get_params(amb_surrogate* s, float* out)
+{
+    clear_error();
+    if (!s || !out) return fail(AMB_E_ARG, "surrogate and out must not be null");
+    SUR_CHECK(hipStreamSynchronize(s->stream));
+    SUR_CHECK(hipMemcpy(out, s->master, s->nparams * 4, hipMemcpyDeviceToHost));
+    return AMB_OK;
+}
+
+int amb_surrogate_set_params(amb_surrogate* s, const float* in)
+{
+    clear_error();
+    if (!s || !in) return fail(AMB_E_ARG, "surrogate and in must not be null");
+    SUR_CHECK(hipStreamSynchronize(s->stream));
+    SUR_CHECK(hipMemcpy(s->master, in, s->nparams * 4, hipMemcpyHostToDevice));
+    const int rc = pack_weights(s);
+    if (rc != AMB_OK) return rc;
+    SUR_CHECK(hipStreamSynchronize(s->stream));
+    return AMB_OK;
+}
+
+int amb_surrogate_loss_grad(amb_surrogate* s, const double* phi, const double* params, const double* target, int batch,
+                            double* loss, float* grad)
+{
+    clear_error();
+    if (!s || !phi || !params || !target) return fail(AMB_E_ARG, "null argument");
+    if (batch < 1 || batch > s->max_batch) return fail(AMB_E_ARG, "batch out of range");
+    int rc = upload_batch(s, batch, phi, params, target);
+    if (rc == AMB_OK) rc = loss_and_grad(s, batch, loss);
+    if (rc != AMB_OK) return rc;
+    if (grad) {
+        SUR_CHECK(hipStreamSynchronize(s->stream));
+        SUR_CHECK(hipMemcpy(grad, s->grad, s->nparams * 4, hipMemcpyDeviceToHost));
+    }
+    return AMB_OK;
+}
+
+int amb_surrogate_set_dataset(amb_surrogate* s, const float* phi_in, const float* phi_out, const float* params,
+                              int64_t count)
+{
+    clear_error();
+    if (!s || !phi_in || !phi_out || !params || count < 1) return fail(AMB_E_ARG, "need a non-empty dataset");
+    SUR_CHECK(hipStreamSynchronize(s->stream));
+    for (float* p : {s->ds_in, s->ds_out, s->ds_par, s->ds_norm})
+        if (p) (void)hipFree(p);
+    s->ds_in = s->ds_out = s->ds_par = s->ds_norm = nullptr;
+    s->ds_count = 0;
+    const size_t hw = (size_t)s->H * s->W;
+    std::vector<float> nm(count);
+    for (int64_t b = 0; b < count; ++b) {
+        double acc = 0;
+        for (size_t i = 0; i < hw; ++i) {
+            const double d = (double)phi_out[b * hw + i] - phi_in[b * hw + i];
+            acc += d * d;
+        }
+        nm[b] = (float)std::max(acc, 1e-30);
+    }
+    SUR_CHECK(hipMalloc(&s->ds_in, count * hw * 4));
+    SUR_CHECK(hipMalloc(&s->ds_out, count * hw * 4));
+    SUR_CHECK(hipMalloc(&s->ds_par, count * 3 * 4));
+    SUR_CHECK(hipMalloc(&s->ds_norm, count * 4));
+    SUR_CHECK(hipMemcpy(s->ds_in, phi_in, count * hw * 4, hipMemcpyHostToDevice));
+    SUR_CHECK(hipMemcpy(s->ds_out, phi_out, count * hw * 4, hipMemcpyHostToDevice));
+    SUR_CHECK(hipMemcpy(s->ds_par, params, count * 3 * 4, hipMemcpyHostToDevice));
+    SUR_CHECK(hipMemcpy(s->ds_norm, nm.data(), count * 4, hipMemcpyHostToDevice));
+    s->ds_count = count;
+    return AMB_OK;
+}
+
+// `iterations` Adam steps on random mini-batches of the dataset (uniform, with
+// replacement across steps, without within a step).  mean_loss: average of the
+// per-step losses (nullable).
+int amb_surrogate_train(amb_surrogate* s, int64_t iterations, int batch, double lr, double clip, double* mean_loss)
+{
+    clear_error();
+    if (!s) return fail(AMB_E_ARG, "surrogate must not be null");
+    if (s->ds_count < batch || batch < 1 || batch > s->max_batch) return fail(AMB_E_ARG, "bad batch or empty dataset");
+    const int hw = s->H * s->W;
+    double total = 0;
+    std::vector<int> idx(batch);
+    for (int64_t it = 0; it < iterations; ++it) {
+        for (int b = 0; b < batch; ++b) {
+            bool dup;
+            do {
+                idx[b] = (int)(s->rng() % (uint64_t)s->ds_count);
+                dup = false;
+                for (int c = 0; c < b; ++c) dup |= idx[c] == idx[b];
+            } while (dup);
+        }
+        SUR_CHECK(hipMemcpyAsync(s->d_idx, idx.data(), batch * sizeof(int), hipMemcpyHostToDevice, s->stream));
+        hipLaunchKernelGGL(gather_kernel, dim3(blocks_for((size_t)hw * batch)), dim3(kEw), 0, s->stream, s->ds_in,
+                           s->d_idx, hw, batch, s->phi);
+        hipLaunchKernelGGL(gather_kernel, dim3(blocks_for((size_t)hw * batch)), dim3(kEw), 0, s->stream, s->ds_out,
+                           s->d_idx, hw, batch, s->target);
+        hipLaunchKernelGGL(gather_kernel, dim3(blocks_for(3 * batch)), dim3(kEw), 0, s->stream, s->ds_par, s->d_idx, 3,
+                           batch, s->params);
+        hipLaunchKernelGGL(gather_kernel, dim3(1), dim3(kEw), 0, s->stream, s->ds_norm, s->d_idx, 1, batch, s->norm);
+        SUR_CHECK(hipGetLastError());
+        double loss = 0;
+        int rc = loss_and_grad(s, batch, mean_loss ? &loss : nullptr);
+        if (rc == AMB_OK) rc = adam_step(s, (float)lr, (float)clip);
+        if (rc != AMB_OK) return rc;
+        total += loss;
+    }
+    SUR_CHECK(hipStreamSynchronize(s->stream));
+    if (mean_loss) *mean_loss = iterations > 0 ? total / iterations : 0.0;
+    return AMB_OK;
+}
+
+// One surrogate step on host fields: phi_next = phi - div J (batch fields of ny*nx).
+int amb_surrogate_predict(amb_surrogate* s, const double* phi, const double* params, int batch, double* phi_next,
+                          double* current)
+{
+    clear_error();
+    if (!s || !phi || !params || !phi_next) return fail(AMB_E_ARG, "null argument");
+    if (batch < 1 || batch > s->max_batch) return fail(AMB_E_ARG, "batch out of range");
+    int rc = upload_batch(s, batch, phi, params, nullptr);
+    if (rc == AMB_OK) rc = forward(s, batch, s->phi, s->params, s->J);
+    if (rc != AMB_OK) return rc;
+    const int P1 = batch * s->H * s->W;
+    hipLaunchKernelGGL(flux_update_kernel, dim3(blocks_for(P1)), dim3(kEw), 0, s->stream, s->phi, s->J, s->H, s->W, P1,
+                       s->pred);
+    SUR_CHECK(hipGetLastError());
+    std::vector<float> out(P1), jj;
+    SUR_CHECK(hipMemcpyAsync(out.data(), s->pred, (size_t)P1 * 4, hipMemcpyDeviceToHost, s->stream));
+    if (current) {
+        jj.resize(2 * (size_t)P1);
+        SUR_CHECK(hipMemcpyAsync(jj.data(), s->J, (size_t)P1 * 8, hipMemcpyDeviceToHost, s->stream));
+    }
+    SUR_CHECK(hipStreamSynchronize(s->stream));
+    for (int i = 0; i < P1; ++i) phi_next[i] = out[i];
+    if (current)
+        for (size_t i = 0; i < jj.size(); ++i) current[i] = jj[i];
+    return AMB_OK;
+}
+
+// nsteps surrogate steps applied in place to the field of an FP32 integrator whose
+// grid matches; its step counter advances by nsteps * euler_steps_per_step.  The
+// per-step launches are captured once in a hipGraph.  device_ms (nullable): HIP-event
+// time of the steps on the integrator's stream.
+int amb_surrogate_advance(amb_surrogate* s, amb_handle* h, int64_t nsteps, const double* params3,
+                          int64_t euler_steps_per_step, double* device_ms)
+{
+    clear_error();
+    if (!s || !h || !params3) return fail(AMB_E_ARG, "null argument");
+    if (nsteps < 0 || euler_steps_per_step < 1) return fail(AMB_E_ARG, "bad step counts");
+    FieldView v;
+    int rc = handle_view(h, &v);
+    if (rc != AMB_OK) return rc;
+    if (v.precision != AMB_F32 || v.ny != s->H || v.nx != s->W)
+        return fail(AMB_E_ARG, "surrogate needs an FP32 integrator of the grid it was built for");
+    // the surrogate works on its own stream; order it after the integrator's work
+    SUR_CHECK(hipStreamSynchronize(v.stream));
+    const float p3[3] = {(float)params3[0], (float)params3[1], (float)params3[2]};
+    SUR_CHECK(hipMemcpyAsync(s->params, p3, sizeof p3, hipMemcpyHostToDevice, s->stream));
+    SUR_CHECK(hipStreamSynchronize(s->stream));
+    const int P1 = s->H * s->W;
+    // Graph of two steps (field -> other -> field) so replays keep the buffer parity.
+    if (!s->step_graph || s->graph_field != v.cur || s->graph_out != v.other) {
+        if (s->step_graph) (void)hipGraphExecDestroy(s->step_graph);
+        s->step_graph = nullptr;
+        hipGraph_t g = nullptr;
+        SUR_CHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        int crc = forward(s, 1, static_cast<float*>(v.cur), s->params, s->J);
+        if (crc == AMB_OK) {
+            hipLaunchKernelGGL(flux_update_kernel, dim3(blocks_for(P1)), dim3(kEw), 0, s->stream,
+                               static_cast<float*>(v.cur), s->J, s->H, s->W, P1, static_cast<float*>(v.other));
+            crc = forward(s, 1, static_cast<float*>(v.other), s->params, s->J);
+        }
+        if (crc == AMB_OK)
+            hipLaunchKernelGGL(flux_update_kernel, dim3(blocks_for(P1)), dim3(kEw), 0, s->stream,
+                               static_cast<float*>(v.other), s->J, s->H, s->W, P1, static_cast<float*>(v.cur));
+        const hipError_t ee = hipStreamEndCapture(s->stream, &g);
+        if (crc != AMB_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return crc;
+        }
+        if (ee != hipSuccess) return fail_hip(ee, "hipStreamEndCapture");
+        const hipError_t ie = hipGraphInstantiate(&s->step_graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) return fail_hip(ie, "hipGraphInstantiate");
+        s->graph_field = v.cur;
+        s->graph_out = v.other;
+    }
+    SUR_CHECK(hipEventRecord(s->ev0, s->stream));
+    for (int64_t k = 0; k + 1 < nsteps; k += 2) SUR_CHECK(hipGraphLaunch(s->step_graph, s->stream));
+    if (nsteps & 1) {
+        rc = forward(s, 1, static_cast<float*>(v.cur), s->params, s->J);
+        if (rc != AMB_OK) return rc;
+        hipLaunchKernelGGL(flux_update_kernel, dim3(blocks_for(P1)), dim3(kEw), 0, s->stream,
+                           static_cast<float*>(v.cur), s->J, s->H, s->W, P1, static_cast<float*>(v.other));
+        SUR_CHECK(hipGetLastError());
+    }
+    SUR_CHECK(hipEventRecord(s->ev1, s->stream));
+    SUR_CHECK(hipEventSynchronize(s->ev1));
+    if (device_ms) {
+        float ms = 0.f;
+        SUR_CHECK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        *device_ms = ms;
+    }
+    // An odd number of steps leaves the field in the other buffer.
+    return handle_commit(h, nsteps * euler_steps_per_step, (nsteps & 1) != 0);
+}
+
+}  // extern "C"

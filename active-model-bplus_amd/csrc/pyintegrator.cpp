@@ -198,6 +198,15 @@ class Integrator {
 
     void set_tile(int index) { check(amb_set_tile(h_, index)); }
 
+    void set_field(const Field& field)
+    {
+        if (field.ndim() != 2 || field.shape(0) != ny_ || field.shape(1) != nx_)
+            throw py::value_error("field must have the integrator's shape");
+        check(amb_set_field(h_, field.data()));
+    }
+
+    amb_handle* handle() const { return h_; }
+
     int autotune(int reps)
     {
         int best = 0, rc;
@@ -214,6 +223,103 @@ class Integrator {
     Model model_;
     py::ssize_t ny_ = 0, nx_ = 0;
     amb_handle* h_ = nullptr;
+};
+
+using F32 = py::array_t<float, py::array::c_style | py::array::forcecast>;
+
+// Learned flux-form surrogate over amb_surrogate_* (no reference counterpart).
+class Surrogate {
+   public:
+    Surrogate(int64_t ny, int64_t nx, int max_batch, uint64_t seed) : ny_(ny), nx_(nx), max_batch_(max_batch)
+    {
+        check(amb_surrogate_create(ny, nx, max_batch, seed, &s_));
+        check(amb_surrogate_param_count(s_, &np_));
+    }
+    ~Surrogate() { amb_surrogate_destroy(s_); }
+    Surrogate(const Surrogate&) = delete;
+    Surrogate& operator=(const Surrogate&) = delete;
+
+    int64_t param_count() const { return np_; }
+
+    py::array_t<float> get_params() const
+    {
+        py::array_t<float> out(np_);
+        check(amb_surrogate_get_params(s_, out.mutable_data()));
+        return out;
+    }
+
+    void set_params(const F32& p)
+    {
+        if (p.size() != np_) throw py::value_error("wrong parameter count");
+        check(amb_surrogate_set_params(s_, p.data()));
+    }
+
+    int batch_of(const Field& phi, const Field& params) const
+    {
+        if (phi.ndim() != 3 || phi.shape(1) != ny_ || phi.shape(2) != nx_)
+            throw py::value_error("fields must have shape (batch, ny, nx)");
+        if (params.ndim() != 2 || params.shape(0) != phi.shape(0) || params.shape(1) != 3)
+            throw py::value_error("params must have shape (batch, 3): lamb, zeta, phi0");
+        return (int)phi.shape(0);
+    }
+
+    py::tuple loss_grad(const Field& phi, const Field& params, const Field& target)
+    {
+        const int b = batch_of(phi, params);
+        if (target.size() != phi.size()) throw py::value_error("target must match phi");
+        double loss = 0;
+        py::array_t<float> grad(np_);
+        check(amb_surrogate_loss_grad(s_, phi.data(), params.data(), target.data(), b, &loss, grad.mutable_data()));
+        return py::make_tuple(loss, grad);
+    }
+
+    void set_dataset(const F32& phi_in, const F32& phi_out, const F32& params)
+    {
+        if (phi_in.ndim() != 3 || phi_in.shape(1) != ny_ || phi_in.shape(2) != nx_ || phi_out.size() != phi_in.size() ||
+            params.size() != 3 * phi_in.shape(0))
+            throw py::value_error("dataset must be (count, ny, nx) x2 and (count, 3)");
+        check(amb_surrogate_set_dataset(s_, phi_in.data(), phi_out.data(), params.data(), phi_in.shape(0)));
+    }
+
+    double train(int64_t iterations, int batch, double lr, double clip)
+    {
+        double loss = 0;
+        int rc;
+        {
+            py::gil_scoped_release release;
+            rc = amb_surrogate_train(s_, iterations, batch, lr, clip, &loss);
+        }
+        check(rc);
+        return loss;
+    }
+
+    py::tuple predict(const Field& phi, const Field& params)
+    {
+        const int b = batch_of(phi, params);
+        py::array_t<double> nxt({(py::ssize_t)b, (py::ssize_t)ny_, (py::ssize_t)nx_});
+        py::array_t<double> cur({(py::ssize_t)b, (py::ssize_t)ny_, (py::ssize_t)nx_, (py::ssize_t)2});
+        check(amb_surrogate_predict(s_, phi.data(), params.data(), b, nxt.mutable_data(), cur.mutable_data()));
+        return py::make_tuple(nxt, cur);
+    }
+
+    double advance(py::object integrator, int64_t nsteps, const std::vector<double>& params3, int64_t euler_steps)
+    {
+        if (params3.size() != 3) throw py::value_error("params3 = (lamb, zeta, phi0)");
+        amb_handle* h = integrator.cast<Integrator&>().handle();
+        double ms = 0;
+        int rc;
+        {
+            py::gil_scoped_release release;
+            rc = amb_surrogate_advance(s_, h, nsteps, params3.data(), euler_steps, &ms);
+        }
+        check(rc);
+        return ms;
+    }
+
+   private:
+    amb_surrogate* s_ = nullptr;
+    int64_t ny_, nx_, np_ = 0;
+    int max_batch_;
 };
 
 // Batched ensemble over amb_batch_* (no reference counterpart).
@@ -404,7 +510,21 @@ PYBIND11_MODULE(integrator, m)
              "test only: explicit unscaled normals (steps, 2, ny, nx) for the next steps")
         .def("set_tile", &Integrator::set_tile, py::arg("index"))
         .def("autotune", &Integrator::autotune, py::arg("reps") = 8,
-             "time every compiled tile shape on a copy of the state and keep the fastest");
+             "time every compiled tile shape on a copy of the state and keep the fastest")
+        .def("set_field", &Integrator::set_field, py::arg("field"), "replace the field (step counter unchanged)");
+
+    py::class_<Surrogate>(m, "Surrogate")
+        .def(py::init<int64_t, int64_t, int, uint64_t>(), py::arg("ny"), py::arg("nx"), py::arg("max_batch") = 16,
+             py::arg("seed") = 0)
+        .def_property_readonly("param_count", &Surrogate::param_count)
+        .def("get_params", &Surrogate::get_params)
+        .def("set_params", &Surrogate::set_params, py::arg("params"))
+        .def("loss_grad", &Surrogate::loss_grad, py::arg("phi"), py::arg("params"), py::arg("target"))
+        .def("set_dataset", &Surrogate::set_dataset, py::arg("phi_in"), py::arg("phi_out"), py::arg("params"))
+        .def("train", &Surrogate::train, py::arg("iterations"), py::arg("batch"), py::arg("lr"), py::arg("clip") = 0.0)
+        .def("predict", &Surrogate::predict, py::arg("phi"), py::arg("params"))
+        .def("advance", &Surrogate::advance, py::arg("integrator"), py::arg("nsteps"), py::arg("params3"),
+             py::arg("euler_steps_per_step"));
 
     py::class_<Ensemble>(m, "Ensemble", py::dynamic_attr())
         .def(py::init<const std::vector<Field>&, const std::vector<Stencil>&, const std::vector<Model>&,

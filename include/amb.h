@@ -94,6 +94,10 @@ int amb_get_field(amb_handle* h, double* out);
 #define AMB_OBS_J_RANDOM 9           /* get_random_current */
 int amb_get_observable(amb_handle* h, int which, double* out);
 
+/* Replaces the current field (ny*nx doubles); the step counter is unchanged.  No
+ * reference counterpart (the reference can only construct a new Integrator). */
+int amb_set_field(amb_handle* h, const double* field);
+
 /* Step counter and parameters.  Replace Integrator::get_timestep, get_stencil,
  * get_model (src/integrator.h:38-39,52-55). */
 int amb_get_timestep(const amb_handle* h, int64_t* timestep);
@@ -163,6 +167,43 @@ int amb_batch_get_tiles(amb_batch* b, int* tile_per_realisation);
  * tile shape and keeps the fastest (state and step counter are restored).
  * `chosen` (nullable) receives the tile index per realisation. */
 int amb_batch_autotune(amb_batch* b, int reps, int* chosen);
+
+/* ---- learned flux-form surrogate (no reference counterpart) --------------- */
+
+/* A periodic U-Net (3 levels, 32/64/128 channels, 3x3 kernels, GELU; BF16 weights
+ * and activations on matrix cores, FP32 accumulation) mapping (phi, lambda, zeta,
+ * phi0) to a current J; one surrogate step is phi <- phi - div J with the
+ * backward-difference divergence, so sum(phi) is conserved exactly.  Grid sides must
+ * be multiples of 64.  Parameters live in one FP32 vector (amb_surrogate_param_count
+ * floats); training uses Adam with optional gradient-norm clipping. */
+typedef struct amb_surrogate amb_surrogate;
+
+int amb_surrogate_create(int64_t ny, int64_t nx, int max_batch, uint64_t seed, amb_surrogate** out);
+int amb_surrogate_destroy(amb_surrogate* s);
+int amb_surrogate_param_count(const amb_surrogate* s, int64_t* n);
+int amb_surrogate_get_params(amb_surrogate* s, float* out);
+int amb_surrogate_set_params(amb_surrogate* s, const float* in);
+
+/* Loss (mean over the batch of |phi - div J - target|^2 / |target - phi|^2) and its
+ * gradient with respect to the parameter vector (grad nullable), for host inputs of
+ * batch fields (ny*nx each) and batch parameter triples (lambda, zeta, phi0). */
+int amb_surrogate_loss_grad(amb_surrogate* s, const double* phi, const double* params, const double* target,
+                            int batch, double* loss, float* grad);
+
+/* Training pairs (count fields each of phi_in and phi_out, FP32, and count triples),
+ * copied to the device; then `iterations` Adam steps on random mini-batches. */
+int amb_surrogate_set_dataset(amb_surrogate* s, const float* phi_in, const float* phi_out, const float* params,
+                              int64_t count);
+int amb_surrogate_train(amb_surrogate* s, int64_t iterations, int batch, double lr, double clip, double* mean_loss);
+
+/* One surrogate step on host fields (current: nullable, 2*ny*nx per field [y, x]). */
+int amb_surrogate_predict(amb_surrogate* s, const double* phi, const double* params, int batch, double* phi_next,
+                          double* current);
+
+/* nsteps surrogate steps in place on an FP32 integrator's field (same grid); its step
+ * counter advances by nsteps * euler_steps_per_step.  device_ms: HIP-event time. */
+int amb_surrogate_advance(amb_surrogate* s, amb_handle* h, int64_t nsteps, const double* params3,
+                          int64_t euler_steps_per_step, double* device_ms);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,107 @@
+"""GPU tests of the learned flux-form surrogate (csrc/amb_unet.hip) against the CPU
+reference network (oracle/unet_oracle.py, which rounds to bfloat16 where the GPU
+stores bfloat16): forward currents, loss, every parameter gradient, exact mass
+conservation, training, and the in-place advance of an integrator's field."""
+import numpy as np
+import pytest
+
+from oracle import unet_oracle as U
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def amb():
+    import activemodelbplus.integrator as m
+    return m
+
+
+def batch(n, h, w, seed):
+    rng = np.random.default_rng(seed)
+    phi = rng.normal(0, 0.3, size=(n, h, w))
+    params = np.stack([rng.uniform(-2, 2, n), rng.uniform(-5, 0, n), rng.uniform(-0.6, 0.6, n)], axis=1)
+    target = phi + 0.05 * rng.normal(size=phi.shape)
+    return phi, params, target
+
+
+def perturbed(S, seed):
+    """Non-trivial parameters (non-zero biases) representable in FP32."""
+    theta = S.get_params().astype(np.float64)
+    theta += 0.03 * np.random.default_rng(seed).normal(size=theta.size)
+    layout, _ = U.param_layout()
+    theta[layout[0][1]:layout[0][2]].reshape(9, 32, 16)[:, :, 4:] = 0
+    S.set_params(theta.astype(np.float32))
+    return S.get_params().astype(np.float64)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def test_forward_matches_oracle(amb):
+    S = amb.Surrogate(64, 64, max_batch=4, seed=1)
+    theta = perturbed(S, 2)
+    phi, params, _ = batch(2, 64, 64, 3)
+    nxt, J = S.predict(phi, params)
+    J_o, nxt_o, _ = U.forward(theta, phi, params, emulate_bf16=True)
+    err_j, err_d = rel(J, J_o), rel(nxt - phi, nxt_o - phi)
+    print(f"rel err J {err_j:.2e}, div J {err_d:.2e}")
+    assert err_j < 2e-3
+    assert err_d < 1e-2  # the divergence differences neighbouring currents
+    # flux form: the sum of the field is unchanged up to FP32 summation
+    assert np.all(np.abs(nxt.sum(axis=(1, 2)) - phi.sum(axis=(1, 2))) < 1e-3)
+
+
+def test_gradients_match_oracle(amb):
+    S = amb.Surrogate(64, 64, max_batch=4, seed=4)
+    theta = perturbed(S, 5)
+    phi, params, target = batch(2, 64, 64, 6)
+    loss, grad = S.loss_grad(phi, params, target)
+    loss_o, grad_o = U.loss_and_grad(theta, phi, params, target, emulate_bf16=True)
+    assert loss == pytest.approx(loss_o, rel=2e-3)
+    layout, _ = U.param_layout()
+    for name, w_off, b_off, cin, cout, taps in layout:
+        gw, gb = grad[w_off:b_off], grad[b_off:b_off + cout]
+        ow, ob = grad_o[w_off:b_off], grad_o[b_off:b_off + cout]
+        print(f"{name}: rel err dW {rel(gw, ow):.2e}  db {rel(gb, ob):.2e}")
+        assert rel(gw, ow) < 2e-2, name
+        if name == "head":
+            # a uniform current has zero divergence: the head bias gradient telescopes to 0
+            assert np.max(np.abs(gb)) < 1e-5 * np.max(np.abs(gw))
+        else:
+            assert rel(gb, ob) < 2e-2, name
+
+
+def test_training_reduces_loss(amb):
+    S = amb.Surrogate(64, 64, max_batch=8, seed=7)
+    phi, params, target = batch(16, 64, 64, 8)
+    # a smooth, learnable target: one explicit diffusion-like step of the field
+    lap = (np.roll(phi, 1, 1) + np.roll(phi, -1, 1) + np.roll(phi, 1, 2) + np.roll(phi, -1, 2) - 4 * phi)
+    target = phi + 0.1 * lap
+    S.set_dataset(phi.astype(np.float32), target.astype(np.float32), params.astype(np.float32))
+    l0 = S.loss_grad(phi[:8], params[:8], target[:8])[0]
+    S.train(300, 8, 2e-3, 1.0)
+    l1 = S.loss_grad(phi[:8], params[:8], target[:8])[0]
+    print(f"loss {l0:.3e} -> {l1:.3e}")
+    assert l1 < 0.3 * l0
+
+
+def test_advance_on_integrator(amb):
+    S = amb.Surrogate(64, 64, max_batch=2, seed=9)
+    perturbed(S, 10)
+    rng = np.random.default_rng(11)
+    phi0 = 0.1 * rng.normal(size=(64, 64))
+    sim = amb.Integrator(phi0, amb.Stencil(5e-3, 1, 1), amb.Model(-0.25, 0, 0.25, 1, 0.5, -1, 0),
+                         seed=1, precision=32)
+    p3 = [0.5, -1.0, 0.0]
+    start = sim.field
+    for n in (3, 2):
+        sim.set_field(start)
+        t0 = sim.timestep
+        ms = S.advance(sim, n, p3, 100)
+        assert ms > 0 and sim.timestep == t0 + 100 * n
+        ref = start.copy()
+        for _ in range(n):
+            ref = S.predict(ref[None], np.array([p3]))[0][0]
+        np.testing.assert_allclose(sim.field, ref, atol=1e-5)
+        assert abs(sim.field.sum() - start.sum()) < 1e-3
