@@ -67,14 +67,71 @@ __device__ __forceinline__ int wrapi(int v, int n) { return v < 0 ? v + n : (v >
 
 __device__ __forceinline__ bf16x8 load8(const bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ u32x4 load8u(const bf16* p) { return *reinterpret_cast<const u32x4*>(p); }
+__device__ __forceinline__ float lo_f(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float hi_f(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
+
 // ---- convolution forward / backward-data --------------------------------------------
 
 enum ConvMode { kOutBf16 = 0, kOutBf16Gelu = 1, kOutF32 = 2 };
+// Where the A operand comes from: a plain NHWC tensor; the 2x2 average of a tensor of
+// twice the resolution (fused avgpool); up2(coarse) + skip (fused up-sampling add);
+// or the network input assembled from phi and the parameters (fused input layer).
+// The fused forms round exactly as the separate kernels do, so results are identical.
+enum ConvSrc { kSrcPlain = 0, kSrcPool = 1, kSrcUpAdd = 2, kSrcInput = 3 };
 
-// out[p][co] = bias[co] + sum_{tap, ci} in[p + shift(tap)][ci] * w[tap][co][ci]
+struct ConvIn {
+    const bf16* a;       // plain / pool (fine tensor) / up-add (coarse tensor)
+    const bf16* skip;    // up-add: fine skip tensor
+    const float* phi;    // input layer
+    const float* params; // input layer: 3 per sample
+};
+
+template <int SRC>
+__device__ __forceinline__ bf16x8 load_a(const ConvIn& src, int cin, int n, int y, int x, int H, int W, int ci)
+{
+    if constexpr (SRC == kSrcPlain) {
+        return load8(src.a + (((size_t)n * H + y) * W + x) * cin + ci);
+    } else if constexpr (SRC == kSrcPool) {
+        // 8 channels as 4 words of two bfloat16 each (element 2j in the low half)
+        const int W2 = 2 * W;
+        const bf16* b = src.a + (((size_t)n * 2 * H + 2 * y) * W2 + 2 * x) * cin + ci;
+        const u32x4 v00 = load8u(b), v01 = load8u(b + cin), v10 = load8u(b + (size_t)W2 * cin),
+                    v11 = load8u(b + (size_t)W2 * cin + cin);
+        u32x4 r;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float lo = lo_f(v00[j]) + lo_f(v01[j]) + lo_f(v10[j]) + lo_f(v11[j]);
+            const float hi = hi_f(v00[j]) + hi_f(v01[j]) + hi_f(v10[j]) + hi_f(v11[j]);
+            r[j] = (uint32_t)f2bf(0.25f * lo) | ((uint32_t)f2bf(0.25f * hi) << 16);
+        }
+        return __builtin_bit_cast(bf16x8, r);
+    } else if constexpr (SRC == kSrcUpAdd) {
+        const u32x4 u = load8u(src.a + (((size_t)n * (H / 2) + y / 2) * (W / 2) + x / 2) * cin + ci);
+        const u32x4 k = load8u(src.skip + (((size_t)n * H + y) * W + x) * cin + ci);
+        u32x4 r;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            r[j] = (uint32_t)f2bf(lo_f(u[j]) + lo_f(k[j])) | ((uint32_t)f2bf(hi_f(u[j]) + hi_f(k[j])) << 16);
+        return __builtin_bit_cast(bf16x8, r);
+    } else {
+        bf16x8 r = {};
+        if (ci == 0) {
+            r[0] = __builtin_bit_cast(__bf16, f2bf(src.phi[((size_t)n * H + y) * W + x]));
+            r[1] = __builtin_bit_cast(__bf16, f2bf(src.params[3 * n + 0] / kParamScale[0]));
+            r[2] = __builtin_bit_cast(__bf16, f2bf(src.params[3 * n + 1] / kParamScale[1]));
+            r[3] = __builtin_bit_cast(__bf16, f2bf(src.params[3 * n + 2] / kParamScale[2]));
+        }
+        return r;
+    }
+}
+
+// out[p][co] = bias[co] + sum_{tap, ci} A[p + shift(tap)][ci] * w[tap][co][ci]
 // (flip: shift negated, used for the data gradient with the [tap][cin][cout] layout).
-template <int MB, int NB, int MODE>
-__global__ __launch_bounds__(256) void conv_kernel(const bf16* __restrict__ in, int cin, const bf16* __restrict__ w,
+// The next K-step's operands are loaded before the current step's MFMAs issue.
+template <int MB, int NB, int MODE, int SRC>
+__global__ __launch_bounds__(256) void conv_kernel(const ConvIn src, int cin, const bf16* __restrict__ w,
                                                    const float* __restrict__ bias, int taps, int flip, int H, int W,
                                                    void* __restrict__ out, bf16* __restrict__ pre)
 {
@@ -98,9 +155,9 @@ __global__ __launch_bounds__(256) void conv_kernel(const bf16* __restrict__ in, 
         for (int n = 0; n < NB; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int K = taps * cin;
     const bf16x8 zero = {};
-    for (int kb = 0; kb < K; kb += 32) {
+    bf16x8 a[MB], b[NB];
+    auto fetch = [&](int kb, bf16x8* av, bf16x8* bv) {
         const int k0 = kb + 8 * q;
-        bf16x8 a[MB], b[NB];
         if (k0 < K) {
             const int tap = k0 / cin, ci = k0 - tap * cin;
             int dy = 0, dx = 0;
@@ -113,22 +170,31 @@ __global__ __launch_bounds__(256) void conv_kernel(const bf16* __restrict__ in, 
                 }
             }
 #pragma unroll
-            for (int m = 0; m < MB; ++m) {
-                const int yy = wrapi(py[m] + dy, H), xx = wrapi(px[m] + dx, W);
-                a[m] = load8(in + (((size_t)pn[m] * H + yy) * W + xx) * cin + ci);
-            }
+            for (int m = 0; m < MB; ++m)
+                av[m] = load_a<SRC>(src, cin, pn[m], wrapi(py[m] + dy, H), wrapi(px[m] + dx, W), H, W, ci);
 #pragma unroll
-            for (int n = 0; n < NB; ++n) b[n] = load8(w + ((size_t)tap * COUT + 16 * n + l16) * cin + ci);
+            for (int n = 0; n < NB; ++n) bv[n] = load8(w + ((size_t)tap * COUT + 16 * n + l16) * cin + ci);
         } else {
 #pragma unroll
-            for (int m = 0; m < MB; ++m) a[m] = zero;
+            for (int m = 0; m < MB; ++m) av[m] = zero;
 #pragma unroll
-            for (int n = 0; n < NB; ++n) b[n] = zero;
+            for (int n = 0; n < NB; ++n) bv[n] = zero;
         }
+    };
+    fetch(0, a, b);
+    for (int kb = 0; kb < K; kb += 32) {
+        bf16x8 an[MB], bn[NB];
+        if (kb + 32 < K) fetch(kb + 32, an, bn);
 #pragma unroll
         for (int m = 0; m < MB; ++m)
 #pragma unroll
             for (int n = 0; n < NB; ++n) acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b[n], acc[m][n], 0, 0, 0);
+        if (kb + 32 < K) {
+#pragma unroll
+            for (int m = 0; m < MB; ++m) a[m] = an[m];
+#pragma unroll
+            for (int n = 0; n < NB; ++n) b[n] = bn[n];
+        }
     }
 #pragma unroll
     for (int m = 0; m < MB; ++m)
@@ -143,13 +209,99 @@ __global__ __launch_bounds__(256) void conv_kernel(const bf16* __restrict__ in, 
                 if constexpr (MODE == kOutF32) {
                     static_cast<float*>(out)[idx] = v;
                 } else if constexpr (MODE == kOutBf16Gelu) {
-                    pre[idx] = f2bf(v);
+                    if (pre) pre[idx] = f2bf(v);
                     static_cast<bf16*>(out)[idx] = f2bf(gelu(v));
                 } else {
                     static_cast<bf16*>(out)[idx] = f2bf(v);
                 }
             }
         }
+}
+
+// Inference convolution for small batches: one block per 16-pixel row tile and all
+// COUT = 16 NB output channels.  The block's KW x NW waves split the K = TAPS CIN
+// reduction (every KW-th 32-wide K-step) and the output channels (NB / NW blocks of
+// 16 each); a wave issues all of its operand loads before its MFMAs, so a launch
+// waits on memory about once.  Partial sums are reduced through LDS in a fixed order.
+// NB2 > 0 fuses a following 1x1 convolution to 16 NB2 channels (no activation): the
+// GELU output tile, rounded to bfloat16 as it would be stored, stays in LDS and only
+// the 1x1 result is written.
+template <int NB, int SRC, int CIN, int TAPS, int KW, int NW, int NB2 = 0>
+__global__ __launch_bounds__(64 * KW * NW) void conv_splitk_kernel(const ConvIn src, const bf16* __restrict__ w,
+                                                                   const float* __restrict__ bias, int H, int W,
+                                                                   bf16* __restrict__ out, const bf16* __restrict__ w2,
+                                                                   const float* __restrict__ b2)
+{
+    constexpr int COUT = 16 * NB, TLD = COUT + 8, K = TAPS * CIN, NS = (K + 31) / 32;
+    constexpr int NBW = NB / NW, SPW = (NS + KW - 1) / KW, NT = 64 * KW * NW;
+    static_assert(NB % NW == 0 && NB2 <= KW * NW && (NB2 == 0 || COUT % 32 == 0), "tile shape");
+    __shared__ float red[KW][16 * COUT];
+    __shared__ __align__(16) bf16 tl[NB2 > 0 ? 16 * TLD : 8];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int kw = wave % KW, nw = wave / KW;
+    const int q = lane >> 4, l16 = lane & 15;
+    const int pix0 = blockIdx.x * 16;
+    const int p = pix0 + l16;
+    const int px = p % W, t = p / W, py = t % H, pn = t / H;
+    const bf16x8 zero = {};
+    bf16x8 a[SPW], b[SPW][NBW];
+#pragma unroll
+    for (int j = 0; j < SPW; ++j) {
+        const int k0 = 32 * (kw + KW * j) + 8 * q;
+        a[j] = zero;
+#pragma unroll
+        for (int n = 0; n < NBW; ++n) b[j][n] = zero;
+        if (k0 < K) {
+            const int tap = k0 / CIN, ci = k0 - tap * CIN;
+            const int dy = TAPS == 9 ? tap / 3 - 1 : 0, dx = TAPS == 9 ? tap % 3 - 1 : 0;
+            a[j] = load_a<SRC>(src, CIN, pn, wrapi(py + dy, H), wrapi(px + dx, W), H, W, ci);
+#pragma unroll
+            for (int n = 0; n < NBW; ++n)
+                b[j][n] = load8(w + ((size_t)tap * COUT + 16 * (nw * NBW + n) + l16) * CIN + ci);
+        }
+    }
+    f32x4 acc[NBW];
+#pragma unroll
+    for (int n = 0; n < NBW; ++n) acc[n] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int j = 0; j < SPW; ++j)
+#pragma unroll
+        for (int n = 0; n < NBW; ++n) acc[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[j], b[j][n], acc[n], 0, 0, 0);
+#pragma unroll
+    for (int n = 0; n < NBW; ++n)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) red[kw][(4 * q + r) * COUT + 16 * (nw * NBW + n) + l16] = acc[n][r];
+    __syncthreads();
+    for (int i = threadIdx.x; i < 16 * COUT; i += NT) {
+        const int co = i % COUT;
+        float v = red[0][i];
+#pragma unroll
+        for (int k = 1; k < KW; ++k) v += red[k][i];
+        v += bias[co];
+        if constexpr (NB2 > 0) {
+            tl[(i / COUT) * TLD + co] = f2bf(gelu(v));
+        } else {
+            out[(size_t)pix0 * COUT + i] = f2bf(gelu(v));
+        }
+    }
+    if constexpr (NB2 > 0) {
+        __syncthreads();
+        if (wave < NB2) {
+            constexpr int C2 = 16 * NB2;
+            bf16x8 a2[COUT / 32], bw[COUT / 32];
+#pragma unroll
+            for (int ks = 0; ks < COUT / 32; ++ks) {
+                a2[ks] = load8(tl + l16 * TLD + 32 * ks + 8 * q);
+                bw[ks] = load8(w2 + (size_t)(16 * wave + l16) * COUT + 32 * ks + 8 * q);
+            }
+            f32x4 d = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < COUT / 32; ++ks) d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[ks], bw[ks], d, 0, 0, 0);
+            const float bb = b2[16 * wave + l16];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) out[(size_t)(pix0 + 4 * q + r) * C2 + 16 * wave + l16] = f2bf(d[r] + bb);
+        }
+    }
 }
 
 // ---- weight gradient ----------------------------------------------------------------
@@ -160,42 +312,52 @@ template <int NBO, int NBI>
 __global__ __launch_bounds__(256) void wgrad_kernel(const bf16* __restrict__ dpre, const bf16* __restrict__ in,
                                                     int taps, int H, int W, int chunk, float* __restrict__ part)
 {
-    constexpr int COUT = 16 * NBO, CIN = 16 * NBI, LD = 40;  // 32 pixels + 8 pad
+    constexpr int COUT = 16 * NBO, CIN = 16 * NBI, PS = 128, LD = PS + 8;  // 128 pixels per stage
     constexpr int TILES = NBO * NBI, TPW = (TILES + 3) / 4;
     __shared__ __align__(16) bf16 sA[COUT * LD];
     __shared__ __align__(16) bf16 sB[CIN * LD];
     const int c = blockIdx.x, tap = blockIdx.y;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int q = lane >> 4, l16 = lane & 15;
-    int dy = 0, dx = 0;
-    if (taps == 9) {
-        dy = tap / 3 - 1;
-        dx = tap % 3 - 1;
-    }
+    const int dy = taps == 9 ? tap / 3 - 1 : 0, dx = taps == 9 ? tap % 3 - 1 : 0;
     f32x4 acc[TPW];
 #pragma unroll
     for (int i = 0; i < TPW; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int pbeg = c * chunk;
-    for (int p0 = pbeg; p0 < pbeg + chunk; p0 += 32) {
-        for (int e = threadIdx.x; e < 32 * COUT; e += 256) {
-            const int pl = e / COUT, ch = e - pl * COUT;
-            sA[ch * LD + pl] = dpre[(size_t)(p0 + pl) * COUT + ch];
+    for (int p0 = pbeg; p0 < pbeg + chunk; p0 += PS) {
+        // 16-byte loads (8 channels of one pixel), written transposed into LDS
+        for (int e = threadIdx.x; e < PS * (COUT / 8); e += 256) {
+            const int pl = e / (COUT / 8), c8 = (e - pl * (COUT / 8)) * 8;
+            const u32x4 v = load8u(dpre + (size_t)(p0 + pl) * COUT + c8);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                sA[(c8 + 2 * j) * LD + pl] = (bf16)(v[j] & 0xffffu);
+                sA[(c8 + 2 * j + 1) * LD + pl] = (bf16)(v[j] >> 16);
+            }
         }
-        for (int e = threadIdx.x; e < 32 * CIN; e += 256) {
-            const int pl = e / CIN, ch = e - pl * CIN;
+        for (int e = threadIdx.x; e < PS * (CIN / 8); e += 256) {
+            const int pl = e / (CIN / 8), c8 = (e - pl * (CIN / 8)) * 8;
             const int p = p0 + pl;
             const int x = p % W, t = p / W, y = t % H, n = t / H;
-            sB[ch * LD + pl] = in[(((size_t)n * H + wrapi(y + dy, H)) * W + wrapi(x + dx, W)) * CIN + ch];
+            const u32x4 v = load8u(in + (((size_t)n * H + wrapi(y + dy, H)) * W + wrapi(x + dx, W)) * CIN + c8);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                sB[(c8 + 2 * j) * LD + pl] = (bf16)(v[j] & 0xffffu);
+                sB[(c8 + 2 * j + 1) * LD + pl] = (bf16)(v[j] >> 16);
+            }
         }
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < TPW; ++i) {
-            const int tile = wave + 4 * i;
-            if (tile < TILES) {
-                const int ib = tile / NBI, jb = tile - ib * NBI;
-                const bf16x8 a = load8(sA + (16 * ib + l16) * LD + 8 * q);
-                const bf16x8 b = load8(sB + (16 * jb + l16) * LD + 8 * q);
-                acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i], 0, 0, 0);
+        for (int ks = 0; ks < PS; ks += 32) {
+#pragma unroll
+            for (int i = 0; i < TPW; ++i) {
+                const int tile = wave + 4 * i;
+                if (tile < TILES) {
+                    const int ib = tile / NBI, jb = tile - ib * NBI;
+                    const bf16x8 a = load8(sA + (16 * ib + l16) * LD + ks + 8 * q);
+                    const bf16x8 b = load8(sB + (16 * jb + l16) * LD + ks + 8 * q);
+                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i], 0, 0, 0);
+                }
             }
         }
         __syncthreads();
@@ -212,25 +374,36 @@ __global__ __launch_bounds__(256) void wgrad_kernel(const bf16* __restrict__ dpr
     }
 }
 
-// Bias gradient partials: part[chunk][co] = sum_{p in chunk} dpre[p][co].
-__global__ void bgrad_kernel(const bf16* __restrict__ dpre, int cout, int chunk, float* __restrict__ part)
+// Bias gradient partials: part[chunk][co] = sum_{p in chunk} dpre[p][co]; threads
+// read consecutive channels of consecutive pixels (coalesced), then reduce in LDS.
+__global__ __launch_bounds__(256) void bgrad_kernel(const bf16* __restrict__ dpre, int cout, int chunk,
+                                                    float* __restrict__ part)
 {
-    const int c = blockIdx.x;
-    for (int co = threadIdx.x; co < cout; co += blockDim.x) {
-        float s = 0.f;
-        for (int p = c * chunk; p < (c + 1) * chunk; ++p) s += bf2f(dpre[(size_t)p * cout + co]);
-        part[(size_t)c * cout + co] = s;
+    __shared__ float red[256];
+    const int c = blockIdx.x, co = threadIdx.x % cout, row = threadIdx.x / cout, rows = 256 / cout;
+    float s = 0.f;
+    for (int p = c * chunk + row; p < (c + 1) * chunk; p += rows) s += bf2f(dpre[(size_t)p * cout + co]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < cout) {
+        float t = 0.f;
+        for (int r = 0; r < rows; ++r) t += red[r * cout + threadIdx.x];
+        part[(size_t)c * cout + threadIdx.x] = t;
     }
 }
 
-// out[i] = sum_c part[c * len + i] (fixed order, deterministic).
-__global__ void reduce_chunks_kernel(const float* __restrict__ part, int nchunks, int len, float* __restrict__ out)
+// out[g * len + i] = sum of part[c * len + i] over chunks c of group g (gsize chunks
+// per group, blockIdx.y = g); fixed order, deterministic.
+__global__ void reduce_chunks_kernel(const float* __restrict__ part, int nchunks, int len, int gsize,
+                                     float* __restrict__ out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= len) return;
+    const int c0 = blockIdx.y * gsize, c1 = min(nchunks, c0 + gsize);
     float s = 0.f;
-    for (int c = 0; c < nchunks; ++c) s += part[(size_t)c * len + i];
-    out[i] = s;
+#pragma unroll 8
+    for (int c = c0; c < c1; ++c) s += part[(size_t)c * len + i];
+    out[(size_t)blockIdx.y * len + i] = s;
 }
 
 // ---- elementwise and resampling ------------------------------------------------------
@@ -305,7 +478,9 @@ __global__ void head_fwd_kernel(const bf16* __restrict__ d1, const float* __rest
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npix) return;
     const int x = p % W, t = p / W, y = t % H, n = t / H;
-    float j0 = bh[0], j1 = bh[1];
+    // one accumulator per tap column, summed as in head_flux_kernel (same rounding)
+    float j0[3] = {bh[0], 0.f, 0.f}, j1[3] = {bh[1], 0.f, 0.f};
+#pragma unroll
     for (int tap = 0; tap < 9; ++tap) {
         const int yy = wrapi(y + tap / 3 - 1, H), xx = wrapi(x + tap % 3 - 1, W);
         const bf16* src = d1 + (((size_t)n * H + yy) * W + xx) * 32;
@@ -313,12 +488,60 @@ __global__ void head_fwd_kernel(const bf16* __restrict__ d1, const float* __rest
 #pragma unroll
         for (int ci = 0; ci < 32; ++ci) {
             const float v = bf2f(src[ci]);
-            j0 = fmaf(v, w0[ci], j0);
-            j1 = fmaf(v, w0[32 + ci], j1);
+            j0[tap % 3] = fmaf(v, w0[ci], j0[tap % 3]);
+            j1[tap % 3] = fmaf(v, w0[32 + ci], j1[tap % 3]);
         }
     }
-    J[2 * (size_t)p] = j0;
-    J[2 * (size_t)p + 1] = j1;
+    J[2 * (size_t)p] = (j0[0] + j0[1]) + j0[2];
+    J[2 * (size_t)p + 1] = (j1[0] + j1[1]) + j1[2];
+}
+
+// Inference: J on a 16x16 tile plus the row above and the column to its left (from a
+// 19x19 patch of d1 in LDS), then phi_out = phi - div J for the tile.
+__global__ __launch_bounds__(320) void head_flux_kernel(const bf16* __restrict__ d1, const float* __restrict__ wh,
+                                                        const float* __restrict__ bh, const float* __restrict__ phi,
+                                                        int H, int W, float* __restrict__ phi_out)
+{
+    __shared__ __align__(16) bf16 sd[19 * 19 * 32];
+    __shared__ float sj[17 * 17 * 2];
+    __shared__ float sw[9 * 64 + 2];
+    const int tilesx = W / 16, tilesy = H / 16;
+    const int b = blockIdx.x, tx = b % tilesx, t2 = b / tilesx, ty = t2 % tilesy, n = t2 / tilesy;
+    const int x0 = tx * 16, y0 = ty * 16;
+    for (int i = threadIdx.x; i < 9 * 64 + 2; i += 320) sw[i] = i < 576 ? wh[i] : bh[i - 576];
+    // d1 patch rows y0-2 .. y0+16, cols x0-2 .. x0+16, 32 channels (4 x 16-byte chunks)
+    for (int i = threadIdx.x; i < 19 * 19 * 4; i += 320) {
+        const int chunk = i & 3, pix = i >> 2, r = pix / 19, c = pix % 19;
+        const int yy = wrapi(y0 - 2 + r, H), xx = wrapi(x0 - 2 + c, W);
+        *reinterpret_cast<bf16x8*>(sd + pix * 32 + 8 * chunk) = load8(d1 + (((size_t)n * H + yy) * W + xx) * 32 + 8 * chunk);
+    }
+    __syncthreads();
+    // J at rows y0-1 .. y0+15, cols x0-1 .. x0+15 (patch coordinates 1 .. 17)
+    if (threadIdx.x < 17 * 17) {
+        const int i = threadIdx.x, r = i / 17 + 1, c = i % 17 + 1;
+        float j0[3] = {sw[576], 0.f, 0.f}, j1[3] = {sw[577], 0.f, 0.f};
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const bf16* src = sd + ((r + tap / 3 - 1) * 19 + (c + tap % 3 - 1)) * 32;
+            const float* w0 = sw + tap * 64;
+#pragma unroll
+            for (int ci = 0; ci < 32; ++ci) {
+                const float v = bf2f(src[ci]);
+                j0[tap % 3] = fmaf(v, w0[ci], j0[tap % 3]);
+                j1[tap % 3] = fmaf(v, w0[32 + ci], j1[tap % 3]);
+            }
+        }
+        sj[2 * i] = (j0[0] + j0[1]) + j0[2];
+        sj[2 * i + 1] = (j1[0] + j1[1]) + j1[2];
+    }
+    __syncthreads();
+    if (threadIdx.x < 256) {
+        const int r = threadIdx.x / 16, c = threadIdx.x % 16;  // tile pixel
+        const int i = (r + 1) * 17 + (c + 1), up = r * 17 + (c + 1), left = (r + 1) * 17 + c;
+        const float div = (sj[2 * i] - sj[2 * up]) + (sj[2 * i + 1] - sj[2 * left + 1]);
+        const size_t p = ((size_t)n * H + y0 + r) * W + x0 + c;
+        phi_out[p] = phi[p] - div;
+    }
 }
 
 // dd1[p][ci] = sum_{tap, c} dJ[p - shift(tap)][c] * wh[tap][c][ci]
@@ -351,13 +574,20 @@ __global__ __launch_bounds__(320) void head_bwd_weight_kernel(const float* __res
     if (t >= 290) return;
     float s0 = 0.f, s1 = 0.f;
     if (t < 288) {
-        const int tap = t / 32, ci = t % 32;
-        for (int p = c * chunk; p < (c + 1) * chunk; ++p) {
-            const int x = p % W, tt = p / W, y = tt % H, n = tt / H;
-            const float v = bf2f(d1[(((size_t)n * H + wrapi(y + tap / 3 - 1, H)) * W + wrapi(x + tap % 3 - 1, W)) * 32 + ci]);
-            s0 = fmaf(dJ[2 * (size_t)p], v, s0);
-            s1 = fmaf(dJ[2 * (size_t)p + 1], v, s1);
+        const int tap = t / 32, ci = t % 32, dy = tap / 3 - 1, dx = tap % 3 - 1;
+        float a0[4] = {0.f, 0.f, 0.f, 0.f}, a1[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int p = c * chunk; p < (c + 1) * chunk; p += 4) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int pp = p + u;
+                const int x = pp % W, tt = pp / W, y = tt % H, n = tt / H;
+                const float v = bf2f(d1[(((size_t)n * H + wrapi(y + dy, H)) * W + wrapi(x + dx, W)) * 32 + ci]);
+                a0[u] = fmaf(dJ[2 * (size_t)pp], v, a0[u]);
+                a1[u] = fmaf(dJ[2 * (size_t)pp + 1], v, a1[u]);
+            }
         }
+        s0 = (a0[0] + a0[1]) + (a0[2] + a0[3]);
+        s1 = (a1[0] + a1[1]) + (a1[2] + a1[3]);
         float* dst = part + (size_t)c * 578;
         dst[tap * 64 + ci] = s0;
         dst[tap * 64 + 32 + ci] = s1;
@@ -502,11 +732,13 @@ constexpr int kEw = 256;  // threads of elementwise kernels
 
 unsigned blocks_for(size_t n, int t = kEw) { return (unsigned)((n + t - 1) / t); }
 
+constexpr int kReduceGroup = 64;  // chunks summed per thread in each level of the reduction
+
 // Pixels per weight-gradient partial: the largest of 4096 .. 256 dividing npix
 // (npix is always a multiple of 256 since grid sides are multiples of 64).
 int chunk_for(int npix)
 {
-    for (int c : {4096, 2048, 1024, 512, 256})
+    for (int c : {4096, 2048, 1024, 512, 256, 128})
         if (npix % c == 0) return c;
     return 0;
 }
@@ -530,8 +762,9 @@ struct amb_surrogate {
     bf16 *s1 = nullptr, *d1 = nullptr, *d1p = nullptr, *dpre = nullptr;
     float *J = nullptr, *phi = nullptr, *target = nullptr, *params = nullptr, *norm = nullptr, *pred = nullptr;
     float *g = nullptr, *dJ = nullptr, *dd1 = nullptr, *ds1 = nullptr, *dd2 = nullptr, *ds2 = nullptr, *dbt = nullptr;
-    float *dp2 = nullptr, *dp1 = nullptr, *part = nullptr, *loss_part = nullptr, *sumsq_part = nullptr;
-    size_t part_len = 0;
+    float *dp2 = nullptr, *dp1 = nullptr, *part = nullptr, *part2 = nullptr, *loss_part = nullptr,
+          *sumsq_part = nullptr;
+    size_t part_len = 0, part2_len = 0;
     int* d_idx = nullptr;
     // dataset (device): phi_in, phi_out [S][H][W], params [S][3], norms [S]
     float *ds_in = nullptr, *ds_out = nullptr, *ds_par = nullptr, *ds_norm = nullptr;
@@ -550,7 +783,7 @@ std::vector<void*> all_buffers(amb_surrogate* s)
     std::vector<void*> v = {s->master, s->grad, s->adam_m, s->adam_v, s->x0, s->e1, s->e1p, s->p1, s->e2, s->e2p,
                             s->p2, s->bt, s->btp, s->u2, s->s2, s->d2, s->d2p, s->u1, s->s1, s->d1, s->d1p, s->dpre,
                             s->J, s->phi, s->target, s->params, s->norm, s->pred, s->g, s->dJ, s->dd1, s->ds1,
-                            s->dd2, s->ds2, s->dbt, s->dp2, s->dp1, s->part, s->loss_part, s->sumsq_part, s->d_idx,
+                            s->dd2, s->ds2, s->dbt, s->dp2, s->dp1, s->part, s->part2, s->loss_part, s->sumsq_part, s->d_idx,
                             s->ds_in, s->ds_out, s->ds_par, s->ds_norm};
     for (int l = 0; l < 7; ++l) {
         v.push_back(s->wf[l]);
@@ -572,28 +805,48 @@ void release(amb_surrogate* s)
     delete s;
 }
 
-template <int MB, int NB, int MODE>
-hipError_t conv_launch(const bf16* in, int cin, const bf16* w, const float* bias, int taps, int flip, int npix, int H,
+template <int MB, int NB, int MODE, int SRC>
+hipError_t conv_launch(const ConvIn& in, int cin, const bf16* w, const float* bias, int taps, int flip, int npix, int H,
                        int W, void* out, bf16* pre, hipStream_t st)
 {
-    hipLaunchKernelGGL((conv_kernel<MB, NB, MODE>), dim3(npix / (64 * MB)), dim3(256), 0, st, in, cin, w, bias, taps,
-                       flip, H, W, out, pre);
+    hipLaunchKernelGGL((conv_kernel<MB, NB, MODE, SRC>), dim3(npix / (64 * MB)), dim3(256), 0, st, in, cin, w, bias,
+                       taps, flip, H, W, out, pre);
     return hipGetLastError();
 }
 
-// Dispatch on output channels (16 NB) with MB chosen to keep 64 accumulator registers.
-template <int MODE>
-hipError_t conv(const bf16* in, int cin, const bf16* w, const float* bias, int cout, int taps, int flip, int npix,
+// Dispatch on output channels (16 NB); MB (16-pixel rows per wave) is 2 for large
+// launches and 1 when the layer has fewer than 2048 pixel rows, to keep the CUs busy.
+template <int NB, int SRC, int CIN, int TAPS, int KW, int NW, int NB2 = 0>
+hipError_t conv_splitk_launch(const ConvIn& in, const bf16* w, const float* bias, int npix, int H, int W, bf16* out,
+                              hipStream_t st, const bf16* w2 = nullptr, const float* b2 = nullptr)
+{
+    hipLaunchKernelGGL((conv_splitk_kernel<NB, SRC, CIN, TAPS, KW, NW, NB2>), dim3(npix / 16), dim3(64 * KW * NW), 0,
+                       st, in, w, bias, H, W, out, w2, b2);
+    return hipGetLastError();
+}
+
+// Dispatch on output channels (16 NB); MB (16-pixel rows per wave) is 2 for large
+// launches and 1 for small.
+template <int MODE, int SRC = kSrcPlain>
+hipError_t conv(const ConvIn& in, int cin, const bf16* w, const float* bias, int cout, int taps, int flip, int npix,
                 int H, int W, void* out, bf16* pre, hipStream_t st)
 {
+    const bool big = npix >= 16 * 2048;
     switch (cout) {
-    case 16: return conv_launch<4, 1, MODE>(in, cin, w, bias, taps, flip, npix, H, W, out, pre, st);
-    case 32: return conv_launch<4, 2, MODE>(in, cin, w, bias, taps, flip, npix, H, W, out, pre, st);
-    case 64: return conv_launch<4, 4, MODE>(in, cin, w, bias, taps, flip, npix, H, W, out, pre, st);
-    case 128: return conv_launch<2, 8, MODE>(in, cin, w, bias, taps, flip, npix, H, W, out, pre, st);
+    case 32:
+        return big ? conv_launch<2, 2, MODE, SRC>(in, cin, w, bias, taps, flip, npix, H, W, out, pre, st)
+                   : conv_launch<1, 2, MODE, SRC>(in, cin, w, bias, taps, flip, npix, H, W, out, pre, st);
+    case 64:
+        return big ? conv_launch<2, 4, MODE, SRC>(in, cin, w, bias, taps, flip, npix, H, W, out, pre, st)
+                   : conv_launch<1, 4, MODE, SRC>(in, cin, w, bias, taps, flip, npix, H, W, out, pre, st);
+    case 128:
+        return big ? conv_launch<2, 8, MODE, SRC>(in, cin, w, bias, taps, flip, npix, H, W, out, pre, st)
+                   : conv_launch<1, 8, MODE, SRC>(in, cin, w, bias, taps, flip, npix, H, W, out, pre, st);
     default: return hipErrorInvalidValue;
     }
 }
+
+ConvIn plain(const bf16* a) { return ConvIn{a, nullptr, nullptr, nullptr}; }
 
 template <int NBO, int NBI>
 hipError_t wgrad_launch(const bf16* dpre, const bf16* in, int taps, int H, int W, int npix, float* part, hipStream_t st)
@@ -620,6 +873,24 @@ hipError_t wgrad(const bf16* dpre, int cout, const bf16* in, int cin, int taps, 
     return hipErrorInvalidValue;
 }
 
+// out[i] = sum over nch chunks of s->part[c * len + i]: one pass for up to 64 chunks,
+// else groups of 64 into s->part2 and then the groups (a tree of fixed shape).
+hipError_t reduce_parts(amb_surrogate* s, int nch, int len, float* out)
+{
+    if (nch <= kReduceGroup) {
+        hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(len), 1), dim3(kEw), 0, s->stream, s->part, nch, len,
+                           nch, out);
+        return hipGetLastError();
+    }
+    const int groups = (nch + kReduceGroup - 1) / kReduceGroup;
+    if ((size_t)groups * len > s->part2_len) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(len), groups), dim3(kEw), 0, s->stream, s->part, nch, len,
+                       kReduceGroup, s->part2);
+    hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(len), 1), dim3(kEw), 0, s->stream, s->part2, groups, len,
+                       groups, out);
+    return hipGetLastError();
+}
+
 // Weight and bias gradient of conv layer l into s->grad.
 int layer_grads(amb_surrogate* s, int l, const bf16* dpre, const bf16* in, int H, int W, int npix)
 {
@@ -627,20 +898,12 @@ int layer_grads(amb_surrogate* s, int l, const bf16* dpre, const bf16* in, int H
     const int chunk = chunk_for(npix), nch = npix / chunk;
     const int len = L.taps * L.cout * L.cin;
     hipError_t e = wgrad(dpre, L.cout, in, L.cin, L.taps, H, W, npix, s->part, s->stream);
+    if (e == hipSuccess) e = reduce_parts(s, nch, len, s->grad + s->w_off[l]);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(len)), dim3(kEw), 0, s->stream, s->part, nch, len,
-                           s->grad + s->w_off[l]);
+        hipLaunchKernelGGL(bgrad_kernel, dim3(npix / 256), dim3(256), 0, s->stream, dpre, L.cout, 256, s->part);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(bgrad_kernel, dim3(nch), dim3(128), 0, s->stream, dpre, L.cout, chunk, s->part);
-        e = hipGetLastError();
-    }
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(L.cout)), dim3(kEw), 0, s->stream, s->part, nch,
-                           L.cout, s->grad + s->b_off[l]);
-        e = hipGetLastError();
-    }
+    if (e == hipSuccess) e = reduce_parts(s, npix / 256, L.cout, s->grad + s->b_off[l]);
     return e == hipSuccess ? AMB_OK : fail_hip(e, "surrogate weight gradient");
 }
 
@@ -673,23 +936,51 @@ int forward(amb_surrogate* s, int n, const float* phi, const float* params, floa
     const float* M = s->master;
     hipLaunchKernelGGL(input_kernel, dim3(blocks_for(P1)), dim3(kEw), 0, st, phi, params, H * W, P1, s->x0);
     SUR_CHECK(hipGetLastError());
-    SUR_CHECK(conv<kOutBf16Gelu>(s->x0, kCin0, s->wf[0], M + s->b_off[0], 32, 9, 0, P1, H, W, s->e1, s->e1p, st));
+    SUR_CHECK(conv<kOutBf16Gelu>(plain(s->x0), kCin0, s->wf[0], M + s->b_off[0], 32, 9, 0, P1, H, W, s->e1, s->e1p, st));
     hipLaunchKernelGGL(avgpool_kernel, dim3(blocks_for((size_t)P2 * 32)), dim3(kEw), 0, st, s->e1, H, W, 32, P2 * 32, s->p1);
     SUR_CHECK(hipGetLastError());
-    SUR_CHECK(conv<kOutBf16Gelu>(s->p1, 32, s->wf[1], M + s->b_off[1], 64, 9, 0, P2, H2, W2, s->e2, s->e2p, st));
+    SUR_CHECK(conv<kOutBf16Gelu>(plain(s->p1), 32, s->wf[1], M + s->b_off[1], 64, 9, 0, P2, H2, W2, s->e2, s->e2p, st));
     hipLaunchKernelGGL(avgpool_kernel, dim3(blocks_for((size_t)P3 * 64)), dim3(kEw), 0, st, s->e2, H2, W2, 64, P3 * 64, s->p2);
     SUR_CHECK(hipGetLastError());
-    SUR_CHECK(conv<kOutBf16Gelu>(s->p2, 64, s->wf[2], M + s->b_off[2], 128, 9, 0, P3, H4, W4, s->bt, s->btp, st));
-    SUR_CHECK(conv<kOutBf16>(s->bt, 128, s->wf[3], M + s->b_off[3], 64, 1, 0, P3, H4, W4, s->u2, nullptr, st));
+    SUR_CHECK(conv<kOutBf16Gelu>(plain(s->p2), 64, s->wf[2], M + s->b_off[2], 128, 9, 0, P3, H4, W4, s->bt, s->btp, st));
+    SUR_CHECK(conv<kOutBf16>(plain(s->bt), 128, s->wf[3], M + s->b_off[3], 64, 1, 0, P3, H4, W4, s->u2, nullptr, st));
     hipLaunchKernelGGL(upadd_kernel, dim3(blocks_for((size_t)P2 * 64)), dim3(kEw), 0, st, s->u2, s->e2, H2, W2, 64, P2 * 64, s->s2);
     SUR_CHECK(hipGetLastError());
-    SUR_CHECK(conv<kOutBf16Gelu>(s->s2, 64, s->wf[4], M + s->b_off[4], 64, 9, 0, P2, H2, W2, s->d2, s->d2p, st));
-    SUR_CHECK(conv<kOutBf16>(s->d2, 64, s->wf[5], M + s->b_off[5], 32, 1, 0, P2, H2, W2, s->u1, nullptr, st));
+    SUR_CHECK(conv<kOutBf16Gelu>(plain(s->s2), 64, s->wf[4], M + s->b_off[4], 64, 9, 0, P2, H2, W2, s->d2, s->d2p, st));
+    SUR_CHECK(conv<kOutBf16>(plain(s->d2), 64, s->wf[5], M + s->b_off[5], 32, 1, 0, P2, H2, W2, s->u1, nullptr, st));
     hipLaunchKernelGGL(upadd_kernel, dim3(blocks_for((size_t)P1 * 32)), dim3(kEw), 0, st, s->u1, s->e1, H, W, 32, P1 * 32, s->s1);
     SUR_CHECK(hipGetLastError());
-    SUR_CHECK(conv<kOutBf16Gelu>(s->s1, 32, s->wf[6], M + s->b_off[6], 32, 9, 0, P1, H, W, s->d1, s->d1p, st));
+    SUR_CHECK(conv<kOutBf16Gelu>(plain(s->s1), 32, s->wf[6], M + s->b_off[6], 32, 9, 0, P1, H, W, s->d1, s->d1p, st));
     hipLaunchKernelGGL(head_fwd_kernel, dim3(blocks_for(P1)), dim3(kEw), 0, st, s->d1, M + s->w_off[7], M + s->b_off[7],
                        H, W, P1, J);
+    SUR_CHECK(hipGetLastError());
+    return AMB_OK;
+}
+
+// Inference: phi_out = phi - div J in six launches, all split-K: pooling, up-sampling
+// adds and the input layer are fused into the convolutions' operand loads, the 1x1
+// layers into the epilogue of the 3x3 layer before them, and the head into the flux
+// update.  The head input d1 stays in memory (predict reads J from it).
+int forward_infer(amb_surrogate* s, int n, const float* phi, const float* params, float* phi_out)
+{
+    const int H = s->H, W = s->W, H2 = H / 2, W2 = W / 2, H4 = H / 4, W4 = W / 4;
+    const int P1 = n * H * W, P2 = n * H2 * W2, P3 = n * H4 * W4;
+    hipStream_t st = s->stream;
+    const float* M = s->master;
+    // <NB, SRC, CIN, TAPS, KW, NW, NB2>
+    SUR_CHECK((conv_splitk_launch<2, kSrcInput, kCin0, 9, 4, 1>(ConvIn{nullptr, nullptr, phi, params}, s->wf[0],
+                                                                 M + s->b_off[0], P1, H, W, s->e1, st)));
+    SUR_CHECK((conv_splitk_launch<4, kSrcPool, 32, 9, 4, 1>(plain(s->e1), s->wf[1], M + s->b_off[1], P2, H2, W2, s->e2,
+                                                             st)));
+    SUR_CHECK((conv_splitk_launch<8, kSrcPool, 64, 9, 4, 1, 4>(plain(s->e2), s->wf[2], M + s->b_off[2], P3, H4, W4,
+                                                                s->u2, st, s->wf[3], M + s->b_off[3])));
+    SUR_CHECK((conv_splitk_launch<4, kSrcUpAdd, 64, 9, 4, 1, 2>(ConvIn{s->u2, s->e2, nullptr, nullptr}, s->wf[4],
+                                                                 M + s->b_off[4], P2, H2, W2, s->u1, st, s->wf[5],
+                                                                 M + s->b_off[5])));
+    SUR_CHECK((conv_splitk_launch<2, kSrcUpAdd, 32, 9, 4, 1>(ConvIn{s->u1, s->e1, nullptr, nullptr}, s->wf[6],
+                                                              M + s->b_off[6], P1, H, W, s->d1, st)));
+    hipLaunchKernelGGL(head_flux_kernel, dim3(n * (H / 16) * (W / 16)), dim3(320), 0, st, s->d1, M + s->w_off[7],
+                       M + s->b_off[7], phi, H, W, phi_out);
     SUR_CHECK(hipGetLastError());
     return AMB_OK;
 }
@@ -712,41 +1003,40 @@ int loss_and_grad(amb_surrogate* s, int n, double* loss_out)
     // head
     hipLaunchKernelGGL(head_bwd_data_kernel, dim3(blocks_for(P1)), dim3(kEw), 0, st, s->dJ, M + s->w_off[7], H, W, P1,
                        s->dd1);
-    const int hchunk = chunk_for(P1), hch = P1 / hchunk;
+    const int hchunk = 256, hch = P1 / hchunk;
     hipLaunchKernelGGL(head_bwd_weight_kernel, dim3(hch), dim3(320), 0, st, s->dJ, s->d1, H, W, hchunk, s->part);
-    hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(578)), dim3(kEw), 0, st, s->part, hch, 578,
-                       s->grad + s->w_off[7]);
     SUR_CHECK(hipGetLastError());
+    SUR_CHECK(reduce_parts(s, hch, 578, s->grad + s->w_off[7]));
     // c7 (in s1), then s1 = up(u1) + e1
     hipLaunchKernelGGL(gelu_bwd_kernel, dim3(blocks_for((size_t)P1 * 32)), dim3(kEw), 0, st, s->dd1, nullptr, s->d1p,
                        H, W, 32, P1 * 32, s->dpre);
     if ((rc = layer_grads(s, 6, s->dpre, s->s1, H, W, P1)) != AMB_OK) return rc;
-    SUR_CHECK(conv<kOutF32>(s->dpre, 32, s->wb[6], nullptr, 32, 9, 1, P1, H, W, s->ds1, nullptr, st));
+    SUR_CHECK(conv<kOutF32>(plain(s->dpre), 32, s->wb[6], nullptr, 32, 9, 1, P1, H, W, s->ds1, nullptr, st));
     hipLaunchKernelGGL(sumpool_kernel, dim3(blocks_for((size_t)P2 * 32)), dim3(kEw), 0, st, s->ds1, H, W, 32, P2 * 32,
                        s->dpre);
     // c6 (1x1, in d2)
     if ((rc = layer_grads(s, 5, s->dpre, s->d2, H2, W2, P2)) != AMB_OK) return rc;
-    SUR_CHECK(conv<kOutF32>(s->dpre, 32, s->wb[5], nullptr, 64, 1, 1, P2, H2, W2, s->dd2, nullptr, st));
+    SUR_CHECK(conv<kOutF32>(plain(s->dpre), 32, s->wb[5], nullptr, 64, 1, 1, P2, H2, W2, s->dd2, nullptr, st));
     // c5 (in s2), then s2 = up(u2) + e2
     hipLaunchKernelGGL(gelu_bwd_kernel, dim3(blocks_for((size_t)P2 * 64)), dim3(kEw), 0, st, s->dd2, nullptr, s->d2p,
                        H2, W2, 64, P2 * 64, s->dpre);
     if ((rc = layer_grads(s, 4, s->dpre, s->s2, H2, W2, P2)) != AMB_OK) return rc;
-    SUR_CHECK(conv<kOutF32>(s->dpre, 64, s->wb[4], nullptr, 64, 9, 1, P2, H2, W2, s->ds2, nullptr, st));
+    SUR_CHECK(conv<kOutF32>(plain(s->dpre), 64, s->wb[4], nullptr, 64, 9, 1, P2, H2, W2, s->ds2, nullptr, st));
     hipLaunchKernelGGL(sumpool_kernel, dim3(blocks_for((size_t)P3 * 64)), dim3(kEw), 0, st, s->ds2, H2, W2, 64, P3 * 64,
                        s->dpre);
     // c4 (1x1, in bt)
     if ((rc = layer_grads(s, 3, s->dpre, s->bt, H4, W4, P3)) != AMB_OK) return rc;
-    SUR_CHECK(conv<kOutF32>(s->dpre, 64, s->wb[3], nullptr, 128, 1, 1, P3, H4, W4, s->dbt, nullptr, st));
+    SUR_CHECK(conv<kOutF32>(plain(s->dpre), 64, s->wb[3], nullptr, 128, 1, 1, P3, H4, W4, s->dbt, nullptr, st));
     // c3 (in p2)
     hipLaunchKernelGGL(gelu_bwd_kernel, dim3(blocks_for((size_t)P3 * 128)), dim3(kEw), 0, st, s->dbt, nullptr, s->btp,
                        H4, W4, 128, P3 * 128, s->dpre);
     if ((rc = layer_grads(s, 2, s->dpre, s->p2, H4, W4, P3)) != AMB_OK) return rc;
-    SUR_CHECK(conv<kOutF32>(s->dpre, 128, s->wb[2], nullptr, 64, 9, 1, P3, H4, W4, s->dp2, nullptr, st));
+    SUR_CHECK(conv<kOutF32>(plain(s->dpre), 128, s->wb[2], nullptr, 64, 9, 1, P3, H4, W4, s->dp2, nullptr, st));
     // c2 (in p1): de2 = ds2 + pool_grad(dp2)
     hipLaunchKernelGGL(gelu_bwd_kernel, dim3(blocks_for((size_t)P2 * 64)), dim3(kEw), 0, st, s->ds2, s->dp2, s->e2p,
                        H2, W2, 64, P2 * 64, s->dpre);
     if ((rc = layer_grads(s, 1, s->dpre, s->p1, H2, W2, P2)) != AMB_OK) return rc;
-    SUR_CHECK(conv<kOutF32>(s->dpre, 64, s->wb[1], nullptr, 32, 9, 1, P2, H2, W2, s->dp1, nullptr, st));
+    SUR_CHECK(conv<kOutF32>(plain(s->dpre), 64, s->wb[1], nullptr, 32, 9, 1, P2, H2, W2, s->dp1, nullptr, st));
     // c1 (in x0): de1 = ds1 + pool_grad(dp1)
     hipLaunchKernelGGL(gelu_bwd_kernel, dim3(blocks_for((size_t)P1 * 32)), dim3(kEw), 0, st, s->ds1, s->dp1, s->e1p,
                        H, W, 32, P1 * 32, s->dpre);
@@ -849,6 +1139,7 @@ int amb_surrogate_create(int64_t ny, int64_t nx, int max_batch, uint64_t seed, a
             const int npix = lv[level_of[l]];
             const size_t len = l == 7 ? 578 : (size_t)kLayers[l].taps * kLayers[l].cout * kLayers[l].cin;
             s->part_len = std::max(s->part_len, (size_t)(npix / chunk_for(npix)) * len);
+            s->part_len = std::max(s->part_len, (size_t)(npix / 256) * (l == 7 ? 578 : kLayers[l].cout));
         }
     }
     hipError_t e = hipSuccess;
@@ -898,6 +1189,8 @@ int amb_surrogate_create(int64_t ny, int64_t nx, int max_batch, uint64_t seed, a
     A(&s->dp2, P3 * 64 * 4);
     A(&s->dp1, P2 * 32 * 4);
     A(&s->part, s->part_len * 4);
+    s->part2_len = s->part_len / kReduceGroup + 9 * 128 * 64;
+    A(&s->part2, s->part2_len * 4);
     A(&s->loss_part, (P1 / 256 + 1) * 4);
     A(&s->sumsq_part, 64 * 4);
     A(&s->d_idx, B * 4);
@@ -1071,13 +1364,16 @@ int amb_surrogate_predict(amb_surrogate* s, const double* phi, const double* par
     clear_error();
     if (!s || !phi || !params || !phi_next) return fail(AMB_E_ARG, "null argument");
     if (batch < 1 || batch > s->max_batch) return fail(AMB_E_ARG, "batch out of range");
+    // the inference kernels of advance(), so both give identical fields
     int rc = upload_batch(s, batch, phi, params, nullptr);
-    if (rc == AMB_OK) rc = forward(s, batch, s->phi, s->params, s->J);
+    if (rc == AMB_OK) rc = forward_infer(s, batch, s->phi, s->params, s->pred);
     if (rc != AMB_OK) return rc;
     const int P1 = batch * s->H * s->W;
-    hipLaunchKernelGGL(flux_update_kernel, dim3(blocks_for(P1)), dim3(kEw), 0, s->stream, s->phi, s->J, s->H, s->W, P1,
-                       s->pred);
-    SUR_CHECK(hipGetLastError());
+    if (current) {
+        hipLaunchKernelGGL(head_fwd_kernel, dim3(blocks_for(P1)), dim3(kEw), 0, s->stream, s->d1, s->master + s->w_off[7],
+                           s->master + s->b_off[7], s->H, s->W, P1, s->J);
+        SUR_CHECK(hipGetLastError());
+    }
     std::vector<float> out(P1), jj;
     SUR_CHECK(hipMemcpyAsync(out.data(), s->pred, (size_t)P1 * 4, hipMemcpyDeviceToHost, s->stream));
     if (current) {
@@ -1118,15 +1414,8 @@ int amb_surrogate_advance(amb_surrogate* s, amb_handle* h, int64_t nsteps, const
         s->step_graph = nullptr;
         hipGraph_t g = nullptr;
         SUR_CHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-        int crc = forward(s, 1, static_cast<float*>(v.cur), s->params, s->J);
-        if (crc == AMB_OK) {
-            hipLaunchKernelGGL(flux_update_kernel, dim3(blocks_for(P1)), dim3(kEw), 0, s->stream,
-                               static_cast<float*>(v.cur), s->J, s->H, s->W, P1, static_cast<float*>(v.other));
-            crc = forward(s, 1, static_cast<float*>(v.other), s->params, s->J);
-        }
-        if (crc == AMB_OK)
-            hipLaunchKernelGGL(flux_update_kernel, dim3(blocks_for(P1)), dim3(kEw), 0, s->stream,
-                               static_cast<float*>(v.other), s->J, s->H, s->W, P1, static_cast<float*>(v.cur));
+        int crc = forward_infer(s, 1, static_cast<float*>(v.cur), s->params, static_cast<float*>(v.other));
+        if (crc == AMB_OK) crc = forward_infer(s, 1, static_cast<float*>(v.other), s->params, static_cast<float*>(v.cur));
         const hipError_t ee = hipStreamEndCapture(s->stream, &g);
         if (crc != AMB_OK) {
             if (g) (void)hipGraphDestroy(g);
@@ -1142,11 +1431,8 @@ int amb_surrogate_advance(amb_surrogate* s, amb_handle* h, int64_t nsteps, const
     SUR_CHECK(hipEventRecord(s->ev0, s->stream));
     for (int64_t k = 0; k + 1 < nsteps; k += 2) SUR_CHECK(hipGraphLaunch(s->step_graph, s->stream));
     if (nsteps & 1) {
-        rc = forward(s, 1, static_cast<float*>(v.cur), s->params, s->J);
+        rc = forward_infer(s, 1, static_cast<float*>(v.cur), s->params, static_cast<float*>(v.other));
         if (rc != AMB_OK) return rc;
-        hipLaunchKernelGGL(flux_update_kernel, dim3(blocks_for(P1)), dim3(kEw), 0, s->stream,
-                           static_cast<float*>(v.cur), s->J, s->H, s->W, P1, static_cast<float*>(v.other));
-        SUR_CHECK(hipGetLastError());
     }
     SUR_CHECK(hipEventRecord(s->ev1, s->stream));
     SUR_CHECK(hipEventSynchronize(s->ev1));
