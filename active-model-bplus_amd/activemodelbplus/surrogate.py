@@ -75,7 +75,8 @@ def initial_fields(rng, params, side):
 
 def generate_pairs(n_real=64, side=256, t_max=1e4, n_times=312, seed=5, log=print):
     """Snapshot pairs (phi(t), phi(t + Delta)) from the batched Euler solver at T = 0,
-    sampled at geometrically spaced times up to t_max.  Returns float32 arrays
+    n_times start times per realisation up to t_max, half geometrically and half
+    uniformly spaced.  Returns float32 arrays
     phi_in, phi_out (S, side, side), params (S, 3) and the sample times (S,)."""
     rng = np.random.default_rng(seed)
     params = sample_params(rng, n_real)
@@ -84,12 +85,16 @@ def generate_pairs(n_real=64, side=256, t_max=1e4, n_times=312, seed=5, log=prin
     ens = Ensemble(fields, Stencil(DT, 1, 1), models, list(range(n_real)), precision=32)
     ens.autotune(2)
     last = int(round(t_max / DT)) - DELTA_STEPS
-    steps = np.unique(np.round(np.geomspace(1, last, n_times)).astype(np.int64))
-    starts, s = [], 0
-    for st in steps:  # keep samples at least Delta apart
-        if not starts or st >= s:
-            starts.append(int(st))
-            s = st + DELTA_STEPS
+    # half geometric (early, fast dynamics), half uniform in time (late coarsening);
+    # consecutive starts at least Delta apart (one trajectory), shifted, not dropped
+    grid = np.sort(np.concatenate([np.geomspace(1, last, n_times - n_times // 2),
+                                   np.linspace(1, last, n_times // 2)]))
+    starts = []
+    for g in grid:
+        st = int(round(g)) if not starts else max(int(round(g)), starts[-1] + DELTA_STEPS)
+        if st > last:
+            break
+        starts.append(st)
     phi_in, phi_out, par, times = [], [], [], []
     cur, t0 = 0, time.time()
     for i, st in enumerate(starts):
@@ -127,6 +132,11 @@ def train(sur, phi_in, phi_out, params, schedule=((4000, 1e-3), (3000, 3e-4), (1
     return hist
 
 
+def default_weights_path():
+    """Weights trained by ``python -m activemodelbplus.surrogate train`` (256 x 256, T = 0)."""
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "surrogate_256.npz")
+
+
 def save_weights(sur, path, meta=None):
     np.savez(path, params=sur.get_params(), meta=json.dumps(meta or {}))
 
@@ -143,11 +153,12 @@ class HybridIntegrator:
     """Advances an FP32 field in coarse steps of ``delta_steps`` Euler steps with the
     surrogate, auditing it against the Euler path every ``audit_every`` coarse steps.
 
-    Audit: from the same state, one surrogate step and ``delta_steps`` Euler steps
-    (on a second integrator, the judge); error = |phi_S - phi_E| / |phi_E| (L2).  If
-    the error exceeds ``tol`` the Euler result is kept and the next ``audit_every``
-    coarse steps run on the Euler path (fallback); otherwise the run continues with
-    the surrogate.  Mass is conserved exactly on both paths.
+    Audit (``Surrogate.audit``, on the device): from the same state, one surrogate
+    step and ``delta_steps`` Euler steps on a second integrator, the judge; error =
+    |phi_S - phi_E| / |phi_E| (L2).  If the error exceeds ``tol`` (or a field is not
+    finite) the Euler result is kept and the next ``audit_every - 1`` coarse steps run
+    on the Euler path (fallback); otherwise the run continues with the surrogate.  Mass
+    is conserved exactly on both paths.
     """
 
     def __init__(self, sur, field, model, stencil=None, delta_steps=DELTA_STEPS, audit_every=20, tol=0.02,
@@ -171,22 +182,14 @@ class HybridIntegrator:
         return self.main.time
 
     def _audit(self):
-        start = self.main.field
-        self.judge.set_field(start)
-        self.judge.run(self.delta)
-        phi_e = self.judge.field
-        self.sur.advance(self.main, 1, self.params3, self.delta)
-        phi_s = self.main.field
-        err = float(np.linalg.norm(phi_s - phi_e) / np.linalg.norm(phi_e))
+        err, ok = self.sur.audit(self.main, self.judge, self.params3, self.delta, self.tol)
         st = self.stats
         st["audits"] += 1
         st["audit_errors"].append(err)
-        st["max_audit_error"] = max(st["max_audit_error"], err)
-        if err > self.tol:
+        st["max_audit_error"] = max(st["max_audit_error"], err) if np.isfinite(err) else float("inf")
+        if not ok:
             st["audit_failures"] += 1
-            self.main.set_field(phi_e)
-            return False
-        return True
+        return ok
 
     def run_coarse(self, n):
         """n coarse steps (n * delta_steps Euler-equivalent steps)."""

@@ -36,7 +36,14 @@ typedef uint16_t bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kCin0 = 16;  // input channels: phi, lamb/2, zeta/5, phi0/0.6, 12 zero pads
+// Input channels: phi rounded to bfloat16, lamb/2, zeta/5, phi0/0.6, and the rounding
+// remainder 256 (phi - bf16(phi)) so that the first convolution sees phi to ~16 bits;
+// 11 zero pads.
+constexpr int kCin0 = 16, kCinReal = 5;
+constexpr float kLoScale = 256.f;
+// The head's output is scaled by kJScale: coarse-step currents are 1e-5 .. 1e-2 while
+// Adam moves weights by ~lr per step, so unscaled head weights could not settle.
+constexpr float kJScale = 1e-3f;
 constexpr float kParamScale[3] = {2.0f, 5.0f, 0.6f};
 constexpr float kGeluC0 = 0.7978845608028654f, kGeluC1 = 0.044715f;
 
@@ -122,6 +129,8 @@ __device__ __forceinline__ bf16x8 load_a(const ConvIn& src, int cin, int n, int 
             r[1] = __builtin_bit_cast(__bf16, f2bf(src.params[3 * n + 0] / kParamScale[0]));
             r[2] = __builtin_bit_cast(__bf16, f2bf(src.params[3 * n + 1] / kParamScale[1]));
             r[3] = __builtin_bit_cast(__bf16, f2bf(src.params[3 * n + 2] / kParamScale[2]));
+            const float f = src.phi[((size_t)n * H + y) * W + x];
+            r[4] = __builtin_bit_cast(__bf16, f2bf((f - bf2f(f2bf(f))) * kLoScale));
         }
         return r;
     }
@@ -218,88 +227,117 @@ __global__ __launch_bounds__(256) void conv_kernel(const ConvIn src, int cin, co
         }
 }
 
-// Inference convolution for small batches: one block per 16-pixel row tile and all
-// COUT = 16 NB output channels.  The block's KW x NW waves split the K = TAPS CIN
-// reduction (every KW-th 32-wide K-step) and the output channels (NB / NW blocks of
-// 16 each); a wave issues all of its operand loads before its MFMAs, so a launch
-// waits on memory about once.  Partial sums are reduced through LDS in a fixed order.
-// NB2 > 0 fuses a following 1x1 convolution to 16 NB2 channels (no activation): the
-// GELU output tile, rounded to bfloat16 as it would be stored, stays in LDS and only
-// the 1x1 result is written.
-template <int NB, int SRC, int CIN, int TAPS, int KW, int NW, int NB2 = 0>
-__global__ __launch_bounds__(64 * KW * NW) void conv_splitk_kernel(const ConvIn src, const bf16* __restrict__ w,
-                                                                   const float* __restrict__ bias, int H, int W,
-                                                                   bf16* __restrict__ out, const bf16* __restrict__ w2,
-                                                                   const float* __restrict__ b2)
+// Inference convolution (3x3, periodic) on a TY x 16 pixel tile per block of four
+// waves.  The block first stages its (TY + 2) x 18 input pixels (halo included) in
+// LDS, applying the operand transform of SRC (input assembly, 2x2 average pooling or
+// up-sampling add) once per pixel; the 3x3 taps then read A fragments from LDS.  The
+// waves form WM row groups x WN channel groups: each accumulates TY / WM row tiles of
+// 16 pixels against 16 NB / WN output channels over all K = 9 CIN, with the weight
+// fragments of up to five K-steps ahead in flight.  Output is
+// GELU(conv + bias) in bfloat16; NB2 > 0 instead applies a fused 1x1 convolution to
+// 16 NB2 channels (no activation) to the bfloat16 GELU tile kept in LDS.
+template <int SRC, int CIN, int NB, int TY, int WM, int WN, int NB2 = 0>
+__global__ __launch_bounds__(256) void conv_tile_kernel(const ConvIn src, const bf16* __restrict__ w,
+                                                        const float* __restrict__ bias, int H, int W,
+                                                        bf16* __restrict__ out, const bf16* __restrict__ w2,
+                                                        const float* __restrict__ b2)
 {
-    constexpr int COUT = 16 * NB, TLD = COUT + 8, K = TAPS * CIN, NS = (K + 31) / 32;
-    constexpr int NBW = NB / NW, SPW = (NS + KW - 1) / KW, NT = 64 * KW * NW;
-    static_assert(NB % NW == 0 && NB2 <= KW * NW && (NB2 == 0 || COUT % 32 == 0), "tile shape");
-    __shared__ float red[KW][16 * COUT];
-    __shared__ __align__(16) bf16 tl[NB2 > 0 ? 16 * TLD : 8];
+    constexpr int COUT = 16 * NB, LDP = CIN + 8, PW = 18, K = 9 * CIN, NS = (K + 31) / 32;
+    constexpr int MB = TY / WM, NBW = NB / WN, TLD = COUT + 8;
+    static_assert(WM * WN == 4 && TY % WM == 0 && NB % WN == 0 && CIN % 8 == 0, "tile shape");
+    static_assert(NB2 == 0 || COUT % 32 == 0, "fused 1x1 needs COUT % 32 == 0");
+    __shared__ __align__(16) bf16 sin[(TY + 2) * PW * LDP];
+    __shared__ __align__(16) bf16 tl[NB2 > 0 ? TY * 16 * TLD : 8];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int kw = wave % KW, nw = wave / KW;
     const int q = lane >> 4, l16 = lane & 15;
-    const int pix0 = blockIdx.x * 16;
-    const int p = pix0 + l16;
-    const int px = p % W, t = p / W, py = t % H, pn = t / H;
+    const int wm = wave % WM, wn = wave / WM;
+    const int tiles_x = W / 16, tiles_y = H / TY;
+    const int bx = blockIdx.x % tiles_x, t = blockIdx.x / tiles_x, by = t % tiles_y, n = t / tiles_y;
+    const int x0 = 16 * bx, y0 = TY * by;
+
+    // stage the padded input tile
+    for (int e = threadIdx.x; e < (TY + 2) * PW * (CIN / 8); e += 256) {
+        const int c8 = e % (CIN / 8), pp = e / (CIN / 8);
+        const int ry = pp / PW, rx = pp - ry * PW;
+        const bf16x8 v = load_a<SRC>(src, CIN, n, wrapi(y0 + ry - 1, H), wrapi(x0 + rx - 1, W), H, W, 8 * c8);
+        *reinterpret_cast<bf16x8*>(sin + pp * LDP + 8 * c8) = v;
+    }
+
     const bf16x8 zero = {};
-    bf16x8 a[SPW], b[SPW][NBW];
+    auto load_b = [&](int st, bf16x8* bb) {
+        const int k0 = 32 * st + 8 * q;
 #pragma unroll
-    for (int j = 0; j < SPW; ++j) {
-        const int k0 = 32 * (kw + KW * j) + 8 * q;
-        a[j] = zero;
-#pragma unroll
-        for (int n = 0; n < NBW; ++n) b[j][n] = zero;
+        for (int j = 0; j < NBW; ++j) bb[j] = zero;
         if (k0 < K) {
             const int tap = k0 / CIN, ci = k0 - tap * CIN;
-            const int dy = TAPS == 9 ? tap / 3 - 1 : 0, dx = TAPS == 9 ? tap % 3 - 1 : 0;
-            a[j] = load_a<SRC>(src, CIN, pn, wrapi(py + dy, H), wrapi(px + dx, W), H, W, ci);
 #pragma unroll
-            for (int n = 0; n < NBW; ++n)
-                b[j][n] = load8(w + ((size_t)tap * COUT + 16 * (nw * NBW + n) + l16) * CIN + ci);
+            for (int j = 0; j < NBW; ++j) bb[j] = load8(w + ((size_t)tap * COUT + 16 * (wn * NBW + j) + l16) * CIN + ci);
+        }
+    };
+    f32x4 acc[MB][NBW];
+#pragma unroll
+    for (int m = 0; m < MB; ++m)
+#pragma unroll
+        for (int j = 0; j < NBW; ++j) acc[m][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // ring of PF K-steps of weight fragments in flight
+    constexpr int PF = NS < 6 ? NS : 6;
+    bf16x8 bq[PF][NBW];
+#pragma unroll
+    for (int i = 0; i < PF - 1; ++i) load_b(i, bq[i]);
+    __syncthreads();
+#pragma unroll
+    for (int st = 0; st < NS; ++st) {
+        if (st + PF - 1 < NS) load_b(st + PF - 1, bq[(st + PF - 1) % PF]);
+        const int k0 = 32 * st + 8 * q;
+        const int tap = k0 / CIN, ci = k0 - tap * CIN;
+        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+#pragma unroll
+        for (int m = 0; m < MB; ++m) {
+            const int r = wm + WM * m;
+            bf16x8 a = zero;
+            if (k0 < K) a = *reinterpret_cast<const bf16x8*>(sin + ((r + 1 + dy) * PW + l16 + 1 + dx) * LDP + ci);
+#pragma unroll
+            for (int j = 0; j < NBW; ++j)
+                acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bq[st % PF][j], acc[m][j], 0, 0, 0);
         }
     }
-    f32x4 acc[NBW];
+
+    // epilogue: D fragment element r of lane (q, l16) is pixel 4q + r, channel l16
 #pragma unroll
-    for (int n = 0; n < NBW; ++n) acc[n] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int m = 0; m < MB; ++m) {
+        const int r = wm + WM * m;
 #pragma unroll
-    for (int j = 0; j < SPW; ++j)
+        for (int j = 0; j < NBW; ++j) {
+            const int co = 16 * (wn * NBW + j) + l16;
+            const float bco = bias[co];
 #pragma unroll
-        for (int n = 0; n < NBW; ++n) acc[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[j], b[j][n], acc[n], 0, 0, 0);
-#pragma unroll
-    for (int n = 0; n < NBW; ++n)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) red[kw][(4 * q + r) * COUT + 16 * (nw * NBW + n) + l16] = acc[n][r];
-    __syncthreads();
-    for (int i = threadIdx.x; i < 16 * COUT; i += NT) {
-        const int co = i % COUT;
-        float v = red[0][i];
-#pragma unroll
-        for (int k = 1; k < KW; ++k) v += red[k][i];
-        v += bias[co];
-        if constexpr (NB2 > 0) {
-            tl[(i / COUT) * TLD + co] = f2bf(gelu(v));
-        } else {
-            out[(size_t)pix0 * COUT + i] = f2bf(gelu(v));
+            for (int e = 0; e < 4; ++e) {
+                const bf16 g = f2bf(gelu(acc[m][j][e] + bco));
+                if constexpr (NB2 > 0) {
+                    tl[(r * 16 + 4 * q + e) * TLD + co] = g;
+                } else {
+                    out[(((size_t)n * H + y0 + r) * W + x0 + 4 * q + e) * COUT + co] = g;
+                }
+            }
         }
     }
     if constexpr (NB2 > 0) {
         __syncthreads();
-        if (wave < NB2) {
-            constexpr int C2 = 16 * NB2;
-            bf16x8 a2[COUT / 32], bw[COUT / 32];
-#pragma unroll
-            for (int ks = 0; ks < COUT / 32; ++ks) {
-                a2[ks] = load8(tl + l16 * TLD + 32 * ks + 8 * q);
-                bw[ks] = load8(w2 + (size_t)(16 * wave + l16) * COUT + 32 * ks + 8 * q);
-            }
+        constexpr int C2 = 16 * NB2;
+        // TY row tiles x NB2 channel blocks of the 1x1 output, spread over the waves
+        for (int job = wave; job < TY * NB2; job += 4) {
+            const int r = job / NB2, j = job % NB2;
             f32x4 d = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int ks = 0; ks < COUT / 32; ++ks) d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[ks], bw[ks], d, 0, 0, 0);
-            const float bb = b2[16 * wave + l16];
+            for (int ks = 0; ks < COUT / 32; ++ks) {
+                const bf16x8 a = *reinterpret_cast<const bf16x8*>(tl + (r * 16 + l16) * TLD + 32 * ks + 8 * q);
+                const bf16x8 b = load8(w2 + (size_t)(16 * j + l16) * COUT + 32 * ks + 8 * q);
+                d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, d, 0, 0, 0);
+            }
+            const float bb = b2[16 * j + l16];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) out[(size_t)(pix0 + 4 * q + r) * C2 + 16 * wave + l16] = f2bf(d[r] + bb);
+            for (int e = 0; e < 4; ++e)
+                out[(((size_t)n * H + y0 + r) * W + x0 + 4 * q + e) * C2 + 16 * j + l16] = f2bf(d[e] + bb);
         }
     }
 }
@@ -419,7 +457,8 @@ __global__ void input_kernel(const float* __restrict__ phi, const float* __restr
     dst[1] = f2bf(params[3 * n + 0] / kParamScale[0]);
     dst[2] = f2bf(params[3 * n + 1] / kParamScale[1]);
     dst[3] = f2bf(params[3 * n + 2] / kParamScale[2]);
-    for (int c = 4; c < kCin0; ++c) dst[c] = 0;
+    dst[4] = f2bf((phi[p] - bf2f(f2bf(phi[p]))) * kLoScale);
+    for (int c = kCinReal; c < kCin0; ++c) dst[c] = 0;
 }
 
 // out (N, H/2, W/2, C) = mean of 2x2 blocks of in (N, H, W, C).
@@ -492,8 +531,8 @@ __global__ void head_fwd_kernel(const bf16* __restrict__ d1, const float* __rest
             j1[tap % 3] = fmaf(v, w0[32 + ci], j1[tap % 3]);
         }
     }
-    J[2 * (size_t)p] = (j0[0] + j0[1]) + j0[2];
-    J[2 * (size_t)p + 1] = (j1[0] + j1[1]) + j1[2];
+    J[2 * (size_t)p] = kJScale * ((j0[0] + j0[1]) + j0[2]);
+    J[2 * (size_t)p + 1] = kJScale * ((j1[0] + j1[1]) + j1[2]);
 }
 
 // Inference: J on a 16x16 tile plus the row above and the column to its left (from a
@@ -502,37 +541,44 @@ __global__ __launch_bounds__(320) void head_flux_kernel(const bf16* __restrict__
                                                         const float* __restrict__ bh, const float* __restrict__ phi,
                                                         int H, int W, float* __restrict__ phi_out)
 {
-    __shared__ __align__(16) bf16 sd[19 * 19 * 32];
+    constexpr int SP = 40;  // LDS pixel stride (32 channels + 8 padding)
+    __shared__ __align__(16) bf16 sd[19 * 19 * SP];
     __shared__ float sj[17 * 17 * 2];
-    __shared__ float sw[9 * 64 + 2];
     const int tilesx = W / 16, tilesy = H / 16;
     const int b = blockIdx.x, tx = b % tilesx, t2 = b / tilesx, ty = t2 % tilesy, n = t2 / tilesy;
     const int x0 = tx * 16, y0 = ty * 16;
-    for (int i = threadIdx.x; i < 9 * 64 + 2; i += 320) sw[i] = i < 576 ? wh[i] : bh[i - 576];
     // d1 patch rows y0-2 .. y0+16, cols x0-2 .. x0+16, 32 channels (4 x 16-byte chunks)
     for (int i = threadIdx.x; i < 19 * 19 * 4; i += 320) {
         const int chunk = i & 3, pix = i >> 2, r = pix / 19, c = pix % 19;
         const int yy = wrapi(y0 - 2 + r, H), xx = wrapi(x0 - 2 + c, W);
-        *reinterpret_cast<bf16x8*>(sd + pix * 32 + 8 * chunk) = load8(d1 + (((size_t)n * H + yy) * W + xx) * 32 + 8 * chunk);
+        *reinterpret_cast<bf16x8*>(sd + pix * SP + 8 * chunk) = load8(d1 + (((size_t)n * H + yy) * W + xx) * 32 + 8 * chunk);
     }
     __syncthreads();
-    // J at rows y0-1 .. y0+15, cols x0-1 .. x0+15 (patch coordinates 1 .. 17)
+    // J at rows y0-1 .. y0+15, cols x0-1 .. x0+15 (patch coordinates 1 .. 17); the
+    // weights are uniform across the block and read as scalars
     if (threadIdx.x < 17 * 17) {
         const int i = threadIdx.x, r = i / 17 + 1, c = i % 17 + 1;
-        float j0[3] = {sw[576], 0.f, 0.f}, j1[3] = {sw[577], 0.f, 0.f};
+        float j0[3] = {bh[0], 0.f, 0.f}, j1[3] = {bh[1], 0.f, 0.f};
 #pragma unroll
         for (int tap = 0; tap < 9; ++tap) {
-            const bf16* src = sd + ((r + tap / 3 - 1) * 19 + (c + tap % 3 - 1)) * 32;
-            const float* w0 = sw + tap * 64;
+            const bf16* src = sd + ((r + tap / 3 - 1) * 19 + (c + tap % 3 - 1)) * SP;
+            const float* w0 = wh + tap * 64;
 #pragma unroll
-            for (int ci = 0; ci < 32; ++ci) {
-                const float v = bf2f(src[ci]);
-                j0[tap % 3] = fmaf(v, w0[ci], j0[tap % 3]);
-                j1[tap % 3] = fmaf(v, w0[32 + ci], j1[tap % 3]);
+            for (int c8 = 0; c8 < 4; ++c8) {
+                const u32x4 v = *reinterpret_cast<const u32x4*>(src + 8 * c8);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int ci = 8 * c8 + 2 * e;
+                    const float lo = lo_f(v[e]), hi = hi_f(v[e]);
+                    j0[tap % 3] = fmaf(lo, w0[ci], j0[tap % 3]);
+                    j1[tap % 3] = fmaf(lo, w0[32 + ci], j1[tap % 3]);
+                    j0[tap % 3] = fmaf(hi, w0[ci + 1], j0[tap % 3]);
+                    j1[tap % 3] = fmaf(hi, w0[33 + ci], j1[tap % 3]);
+                }
             }
         }
-        sj[2 * i] = (j0[0] + j0[1]) + j0[2];
-        sj[2 * i + 1] = (j1[0] + j1[1]) + j1[2];
+        sj[2 * i] = kJScale * ((j0[0] + j0[1]) + j0[2]);
+        sj[2 * i + 1] = kJScale * ((j1[0] + j1[1]) + j1[2]);
     }
     __syncthreads();
     if (threadIdx.x < 256) {
@@ -613,6 +659,36 @@ __global__ void flux_update_kernel(const float* __restrict__ phi, const float* _
     out[p] = phi[p] - div;
 }
 
+constexpr int kAuditBlocks = 256;
+
+// part[2 b] = sum (a - e)^2, part[2 b + 1] = sum e^2 over block b's grid-stride share,
+// in FP64 (non-finite values propagate into the sums).
+__global__ __launch_bounds__(256) void audit_kernel(const float* __restrict__ a, const float* __restrict__ e, int n,
+                                                    double* __restrict__ part)
+{
+    __shared__ double r0[256], r1[256];
+    double s0 = 0.0, s1 = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kAuditBlocks * 256) {
+        const double d = (double)a[i] - (double)e[i];
+        s0 += d * d;
+        s1 += (double)e[i] * e[i];
+    }
+    r0[threadIdx.x] = s0;
+    r1[threadIdx.x] = s1;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            r0[threadIdx.x] += r0[threadIdx.x + w];
+            r1[threadIdx.x] += r1[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = r0[0];
+        part[2 * blockIdx.x + 1] = r1[0];
+    }
+}
+
 // g = 2 (pred - target) / (norm_n N); part[block] = sum of (pred - target)^2 / norm_n.
 __global__ __launch_bounds__(256) void loss_kernel(const float* __restrict__ pred, const float* __restrict__ target,
                                                    const float* __restrict__ norm, int hw, int npix, int batch,
@@ -636,15 +712,16 @@ __global__ __launch_bounds__(256) void loss_kernel(const float* __restrict__ pre
     if (threadIdx.x == 0) part[blockIdx.x] = s[0];
 }
 
-// dJ = -div^T g: dJy(i,j) = g(i+1,j) - g(i,j), dJx(i,j) = g(i,j+1) - g(i,j).
+// Gradient with respect to the unscaled head output: kJScale (-div^T g), where
+// (-div^T g)_y(i,j) = g(i+1,j) - g(i,j) and (-div^T g)_x(i,j) = g(i,j+1) - g(i,j).
 __global__ void dj_kernel(const float* __restrict__ g, int H, int W, int npix, float* __restrict__ dJ)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npix) return;
     const int x = p % W, t = p / W, y = t % H, n = t / H;
     const float gp = g[p];
-    dJ[2 * (size_t)p] = g[((size_t)n * H + wrapi(y + 1, H)) * W + x] - gp;
-    dJ[2 * (size_t)p + 1] = g[((size_t)n * H + y) * W + wrapi(x + 1, W)] - gp;
+    dJ[2 * (size_t)p] = kJScale * (g[((size_t)n * H + wrapi(y + 1, H)) * W + x] - gp);
+    dJ[2 * (size_t)p + 1] = kJScale * (g[((size_t)n * H + y) * W + wrapi(x + 1, W)] - gp);
 }
 
 // ---- optimiser -------------------------------------------------------------------------
@@ -765,6 +842,9 @@ struct amb_surrogate {
     float *dp2 = nullptr, *dp1 = nullptr, *part = nullptr, *part2 = nullptr, *loss_part = nullptr,
           *sumsq_part = nullptr;
     size_t part_len = 0, part2_len = 0;
+    double* audit_part = nullptr;  // kAuditBlocks x (|a - b|^2, |b|^2)
+    float* p3 = nullptr;           // parameter triple of advance / audit (device)
+    float last_p3[3] = {NAN, NAN, NAN};
     int* d_idx = nullptr;
     // dataset (device): phi_in, phi_out [S][H][W], params [S][3], norms [S]
     float *ds_in = nullptr, *ds_out = nullptr, *ds_par = nullptr, *ds_norm = nullptr;
@@ -783,7 +863,7 @@ std::vector<void*> all_buffers(amb_surrogate* s)
     std::vector<void*> v = {s->master, s->grad, s->adam_m, s->adam_v, s->x0, s->e1, s->e1p, s->p1, s->e2, s->e2p,
                             s->p2, s->bt, s->btp, s->u2, s->s2, s->d2, s->d2p, s->u1, s->s1, s->d1, s->d1p, s->dpre,
                             s->J, s->phi, s->target, s->params, s->norm, s->pred, s->g, s->dJ, s->dd1, s->ds1,
-                            s->dd2, s->ds2, s->dbt, s->dp2, s->dp1, s->part, s->part2, s->loss_part, s->sumsq_part, s->d_idx,
+                            s->dd2, s->ds2, s->dbt, s->dp2, s->dp1, s->part, s->part2, s->audit_part, s->p3, s->loss_part, s->sumsq_part, s->d_idx,
                             s->ds_in, s->ds_out, s->ds_par, s->ds_norm};
     for (int l = 0; l < 7; ++l) {
         v.push_back(s->wf[l]);
@@ -814,13 +894,12 @@ hipError_t conv_launch(const ConvIn& in, int cin, const bf16* w, const float* bi
     return hipGetLastError();
 }
 
-// Dispatch on output channels (16 NB); MB (16-pixel rows per wave) is 2 for large
-// launches and 1 when the layer has fewer than 2048 pixel rows, to keep the CUs busy.
-template <int NB, int SRC, int CIN, int TAPS, int KW, int NW, int NB2 = 0>
-hipError_t conv_splitk_launch(const ConvIn& in, const bf16* w, const float* bias, int npix, int H, int W, bf16* out,
-                              hipStream_t st, const bf16* w2 = nullptr, const float* b2 = nullptr)
+template <int SRC, int CIN, int NB, int TY, int WM, int WN, int NB2 = 0>
+hipError_t conv_tile_launch(const ConvIn& in, const bf16* w, const float* bias, int n, int H, int W, bf16* out,
+                            hipStream_t st, const bf16* w2 = nullptr, const float* b2 = nullptr)
 {
-    hipLaunchKernelGGL((conv_splitk_kernel<NB, SRC, CIN, TAPS, KW, NW, NB2>), dim3(npix / 16), dim3(64 * KW * NW), 0,
+    if (H % TY || W % 16) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((conv_tile_kernel<SRC, CIN, NB, TY, WM, WN, NB2>), dim3(n * (H / TY) * (W / 16)), dim3(256), 0,
                        st, in, w, bias, H, W, out, w2, b2);
     return hipGetLastError();
 }
@@ -967,18 +1046,18 @@ int forward_infer(amb_surrogate* s, int n, const float* phi, const float* params
     const int P1 = n * H * W, P2 = n * H2 * W2, P3 = n * H4 * W4;
     hipStream_t st = s->stream;
     const float* M = s->master;
-    // <NB, SRC, CIN, TAPS, KW, NW, NB2>
-    SUR_CHECK((conv_splitk_launch<2, kSrcInput, kCin0, 9, 4, 1>(ConvIn{nullptr, nullptr, phi, params}, s->wf[0],
-                                                                 M + s->b_off[0], P1, H, W, s->e1, st)));
-    SUR_CHECK((conv_splitk_launch<4, kSrcPool, 32, 9, 4, 1>(plain(s->e1), s->wf[1], M + s->b_off[1], P2, H2, W2, s->e2,
-                                                             st)));
-    SUR_CHECK((conv_splitk_launch<8, kSrcPool, 64, 9, 4, 1, 4>(plain(s->e2), s->wf[2], M + s->b_off[2], P3, H4, W4,
-                                                                s->u2, st, s->wf[3], M + s->b_off[3])));
-    SUR_CHECK((conv_splitk_launch<4, kSrcUpAdd, 64, 9, 4, 1, 2>(ConvIn{s->u2, s->e2, nullptr, nullptr}, s->wf[4],
-                                                                 M + s->b_off[4], P2, H2, W2, s->u1, st, s->wf[5],
-                                                                 M + s->b_off[5])));
-    SUR_CHECK((conv_splitk_launch<2, kSrcUpAdd, 32, 9, 4, 1>(ConvIn{s->u1, s->e1, nullptr, nullptr}, s->wf[6],
-                                                              M + s->b_off[6], P1, H, W, s->d1, st)));
+    // <SRC, CIN, NB, TY, WM, WN, NB2>
+    SUR_CHECK((conv_tile_launch<kSrcInput, kCin0, 2, 8, 4, 1>(ConvIn{nullptr, nullptr, phi, params}, s->wf[0],
+                                                               M + s->b_off[0], n, H, W, s->e1, st)));
+    SUR_CHECK((conv_tile_launch<kSrcPool, 32, 4, 4, 4, 1>(plain(s->e1), s->wf[1], M + s->b_off[1], n, H2, W2, s->e2,
+                                                           st)));
+    SUR_CHECK((conv_tile_launch<kSrcPool, 64, 8, 2, 2, 2, 4>(plain(s->e2), s->wf[2], M + s->b_off[2], n, H4, W4, s->u2,
+                                                              st, s->wf[3], M + s->b_off[3])));
+    SUR_CHECK((conv_tile_launch<kSrcUpAdd, 64, 4, 4, 4, 1, 2>(ConvIn{s->u2, s->e2, nullptr, nullptr}, s->wf[4],
+                                                               M + s->b_off[4], n, H2, W2, s->u1, st, s->wf[5],
+                                                               M + s->b_off[5])));
+    SUR_CHECK((conv_tile_launch<kSrcUpAdd, 32, 2, 8, 4, 1>(ConvIn{s->u1, s->e1, nullptr, nullptr}, s->wf[6],
+                                                            M + s->b_off[6], n, H, W, s->d1, st)));
     hipLaunchKernelGGL(head_flux_kernel, dim3(n * (H / 16) * (W / 16)), dim3(320), 0, st, s->d1, M + s->w_off[7],
                        M + s->b_off[7], phi, H, W, phi_out);
     SUR_CHECK(hipGetLastError());
@@ -1074,6 +1153,12 @@ int check_dims(int64_t ny, int64_t nx)
     return AMB_OK;
 }
 
+// Per-sample loss normaliser |target - phi|^2 + kLossFloor * (grid sites): the floor
+// keeps samples whose field barely changes over a coarse step (uniform fields outside
+// the spinodal region) from dominating the loss.
+constexpr double kLossFloor = 1e-8;
+float loss_norm(double sq, size_t hw) { return (float)(sq + kLossFloor * (double)hw); }
+
 // Upload host doubles for a batch into phi/params/target/norm.
 int upload_batch(amb_surrogate* s, int n, const double* phi, const double* params, const double* target)
 {
@@ -1093,7 +1178,7 @@ int upload_batch(amb_surrogate* s, int n, const double* phi, const double* param
                 const double d = target[b * hw + i] - phi[b * hw + i];
                 acc += d * d;
             }
-            nm[b] = (float)std::max(acc, 1e-30);
+            nm[b] = loss_norm(acc, hw);
         }
         SUR_CHECK(hipMemcpyAsync(s->target, t.data(), t.size() * 4, hipMemcpyHostToDevice, s->stream));
         SUR_CHECK(hipMemcpyAsync(s->norm, nm.data(), nm.size() * 4, hipMemcpyHostToDevice, s->stream));
@@ -1191,6 +1276,8 @@ int amb_surrogate_create(int64_t ny, int64_t nx, int max_batch, uint64_t seed, a
     A(&s->part, s->part_len * 4);
     s->part2_len = s->part_len / kReduceGroup + 9 * 128 * 64;
     A(&s->part2, s->part2_len * 4);
+    A(&s->audit_part, kAuditBlocks * 2 * sizeof(double));
+    A(&s->p3, 4 * sizeof(float));
     A(&s->loss_part, (P1 / 256 + 1) * 4);
     A(&s->sumsq_part, 64 * 4);
     A(&s->d_idx, B * 4);
@@ -1208,14 +1295,14 @@ int amb_surrogate_create(int64_t ny, int64_t nx, int max_batch, uint64_t seed, a
     std::uniform_real_distribution<double> U(-1.0, 1.0);
     for (int l = 0; l < 8; ++l) {
         const LayerDef& L = kLayers[l];
-        const int real_cin = l == 0 ? 4 : L.cin;
+        const int real_cin = l == 0 ? kCinReal : L.cin;
         double bound = std::sqrt(6.0 / (L.taps * real_cin));
         if (l == 7) bound *= 0.1;
         for (int t = 0; t < L.taps; ++t)
             for (int co = 0; co < L.cout; ++co)
                 for (int ci = 0; ci < L.cin; ++ci)
                     init[s->w_off[l] + ((size_t)t * L.cout + co) * L.cin + ci] =
-                        (l == 0 && ci >= 4) ? 0.f : (float)(bound * U(g));
+                        (l == 0 && ci >= kCinReal) ? 0.f : (float)(bound * U(g));
     }
     std::vector<float> zeros(s->nparams, 0.f);
     e = hipMemcpy(s->master, init.data(), s->nparams * 4, hipMemcpyHostToDevice);
@@ -1303,7 +1390,7 @@ int amb_surrogate_set_dataset(amb_surrogate* s, const float* phi_in, const float
             const double d = (double)phi_out[b * hw + i] - phi_in[b * hw + i];
             acc += d * d;
         }
-        nm[b] = (float)std::max(acc, 1e-30);
+        nm[b] = loss_norm(acc, hw);
     }
     SUR_CHECK(hipMalloc(&s->ds_in, count * hw * 4));
     SUR_CHECK(hipMalloc(&s->ds_out, count * hw * 4));
@@ -1387,6 +1474,76 @@ int amb_surrogate_predict(amb_surrogate* s, const double* phi, const double* par
     return AMB_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Parameter triple of advance / audit into s->p3 (skipped when unchanged).
+int upload_params3(amb_surrogate* s, const double* params3)
+{
+    const float p3[3] = {(float)params3[0], (float)params3[1], (float)params3[2]};
+    if (p3[0] == s->last_p3[0] && p3[1] == s->last_p3[1] && p3[2] == s->last_p3[2]) return AMB_OK;
+    SUR_CHECK(hipMemcpyAsync(s->p3, p3, sizeof p3, hipMemcpyHostToDevice, s->stream));
+    SUR_CHECK(hipStreamSynchronize(s->stream));
+    for (int i = 0; i < 3; ++i) s->last_p3[i] = p3[i];
+    return AMB_OK;
+}
+
+int check_surrogate_view(const amb_surrogate* s, const FieldView& v)
+{
+    if (v.precision != AMB_F32 || v.ny != s->H || v.nx != s->W)
+        return fail(AMB_E_ARG, "surrogate needs an FP32 integrator of the grid it was built for");
+    return AMB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int amb_surrogate_audit(amb_surrogate* s, amb_handle* main, amb_handle* judge, const double* params3,
+                        int64_t euler_steps, double tol, double* rel_err, int* accepted)
+{
+    clear_error();
+    if (!s || !main || !judge || !params3 || !rel_err || !accepted) return fail(AMB_E_ARG, "null argument");
+    if (main == judge) return fail(AMB_E_ARG, "main and judge must be different integrators");
+    if (euler_steps < 1) return fail(AMB_E_ARG, "euler_steps must be positive");
+    FieldView mv, jv;
+    int rc = handle_view(main, &mv);
+    if (rc == AMB_OK) rc = handle_view(judge, &jv);
+    if (rc == AMB_OK) rc = check_surrogate_view(s, mv);
+    if (rc == AMB_OK) rc = check_surrogate_view(s, jv);
+    if (rc != AMB_OK) return rc;
+    const size_t bytes = (size_t)s->H * s->W * sizeof(float);
+    SUR_CHECK(hipStreamSynchronize(mv.stream));
+    SUR_CHECK(hipStreamSynchronize(jv.stream));
+    // judge: the same start, euler_steps explicit-Euler steps
+    SUR_CHECK(hipMemcpyAsync(jv.cur, mv.cur, bytes, hipMemcpyDeviceToDevice, jv.stream));
+    if ((rc = amb_run(judge, euler_steps)) != AMB_OK) return rc;
+    if ((rc = handle_view(judge, &jv)) != AMB_OK) return rc;
+    // surrogate: one step into main's other buffer
+    if ((rc = upload_params3(s, params3)) != AMB_OK) return rc;
+    if ((rc = forward_infer(s, 1, static_cast<float*>(mv.cur), s->p3, static_cast<float*>(mv.other))) != AMB_OK)
+        return rc;
+    hipLaunchKernelGGL(audit_kernel, dim3(kAuditBlocks), dim3(256), 0, s->stream, static_cast<const float*>(mv.other),
+                       static_cast<const float*>(jv.cur), s->H * s->W, s->audit_part);
+    SUR_CHECK(hipGetLastError());
+    double part[2 * kAuditBlocks];
+    SUR_CHECK(hipMemcpyAsync(part, s->audit_part, sizeof part, hipMemcpyDeviceToHost, s->stream));
+    SUR_CHECK(hipStreamSynchronize(s->stream));
+    double num = 0.0, den = 0.0;
+    for (int b = 0; b < kAuditBlocks; ++b) {
+        num += part[2 * b];
+        den += part[2 * b + 1];
+    }
+    const double err = std::sqrt(num) / std::sqrt(den);
+    *rel_err = err;
+    *accepted = err <= tol ? 1 : 0;  // false for NaN
+    if (*accepted) return handle_commit(main, euler_steps, true);
+    SUR_CHECK(hipMemcpyAsync(mv.cur, jv.cur, bytes, hipMemcpyDeviceToDevice, s->stream));
+    SUR_CHECK(hipStreamSynchronize(s->stream));
+    return handle_commit(main, euler_steps, false);
+}
+
 // nsteps surrogate steps applied in place to the field of an FP32 integrator whose
 // grid matches; its step counter advances by nsteps * euler_steps_per_step.  The
 // per-step launches are captured once in a hipGraph.  device_ms (nullable): HIP-event
@@ -1399,14 +1556,11 @@ int amb_surrogate_advance(amb_surrogate* s, amb_handle* h, int64_t nsteps, const
     if (nsteps < 0 || euler_steps_per_step < 1) return fail(AMB_E_ARG, "bad step counts");
     FieldView v;
     int rc = handle_view(h, &v);
+    if (rc == AMB_OK) rc = check_surrogate_view(s, v);
     if (rc != AMB_OK) return rc;
-    if (v.precision != AMB_F32 || v.ny != s->H || v.nx != s->W)
-        return fail(AMB_E_ARG, "surrogate needs an FP32 integrator of the grid it was built for");
     // the surrogate works on its own stream; order it after the integrator's work
     SUR_CHECK(hipStreamSynchronize(v.stream));
-    const float p3[3] = {(float)params3[0], (float)params3[1], (float)params3[2]};
-    SUR_CHECK(hipMemcpyAsync(s->params, p3, sizeof p3, hipMemcpyHostToDevice, s->stream));
-    SUR_CHECK(hipStreamSynchronize(s->stream));
+    if ((rc = upload_params3(s, params3)) != AMB_OK) return rc;
     const int P1 = s->H * s->W;
     // Graph of two steps (field -> other -> field) so replays keep the buffer parity.
     if (!s->step_graph || s->graph_field != v.cur || s->graph_out != v.other) {
@@ -1414,8 +1568,8 @@ int amb_surrogate_advance(amb_surrogate* s, amb_handle* h, int64_t nsteps, const
         s->step_graph = nullptr;
         hipGraph_t g = nullptr;
         SUR_CHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-        int crc = forward_infer(s, 1, static_cast<float*>(v.cur), s->params, static_cast<float*>(v.other));
-        if (crc == AMB_OK) crc = forward_infer(s, 1, static_cast<float*>(v.other), s->params, static_cast<float*>(v.cur));
+        int crc = forward_infer(s, 1, static_cast<float*>(v.cur), s->p3, static_cast<float*>(v.other));
+        if (crc == AMB_OK) crc = forward_infer(s, 1, static_cast<float*>(v.other), s->p3, static_cast<float*>(v.cur));
         const hipError_t ee = hipStreamEndCapture(s->stream, &g);
         if (crc != AMB_OK) {
             if (g) (void)hipGraphDestroy(g);
@@ -1431,7 +1585,7 @@ int amb_surrogate_advance(amb_surrogate* s, amb_handle* h, int64_t nsteps, const
     SUR_CHECK(hipEventRecord(s->ev0, s->stream));
     for (int64_t k = 0; k + 1 < nsteps; k += 2) SUR_CHECK(hipGraphLaunch(s->step_graph, s->stream));
     if (nsteps & 1) {
-        rc = forward_infer(s, 1, static_cast<float*>(v.cur), s->params, static_cast<float*>(v.other));
+        rc = forward_infer(s, 1, static_cast<float*>(v.cur), s->p3, static_cast<float*>(v.other));
         if (rc != AMB_OK) return rc;
     }
     SUR_CHECK(hipEventRecord(s->ev1, s->stream));

@@ -316,6 +316,22 @@ class Surrogate {
         return ms;
     }
 
+    py::tuple audit(py::object main, py::object judge, const std::vector<double>& params3, int64_t euler_steps,
+                    double tol)
+    {
+        if (params3.size() != 3) throw py::value_error("params3 = (lamb, zeta, phi0)");
+        amb_handle* hm = main.cast<Integrator&>().handle();
+        amb_handle* hj = judge.cast<Integrator&>().handle();
+        double err = 0;
+        int accepted = 0, rc;
+        {
+            py::gil_scoped_release release;
+            rc = amb_surrogate_audit(s_, hm, hj, params3.data(), euler_steps, tol, &err, &accepted);
+        }
+        check(rc);
+        return py::make_tuple(err, accepted != 0);
+    }
+
    private:
     amb_surrogate* s_ = nullptr;
     int64_t ny_, nx_, np_ = 0;
@@ -524,7 +540,9 @@ PYBIND11_MODULE(integrator, m)
         .def("train", &Surrogate::train, py::arg("iterations"), py::arg("batch"), py::arg("lr"), py::arg("clip") = 0.0)
         .def("predict", &Surrogate::predict, py::arg("phi"), py::arg("params"))
         .def("advance", &Surrogate::advance, py::arg("integrator"), py::arg("nsteps"), py::arg("params3"),
-             py::arg("euler_steps_per_step"));
+             py::arg("euler_steps_per_step"))
+        .def("audit", &Surrogate::audit, py::arg("main"), py::arg("judge"), py::arg("params3"),
+             py::arg("euler_steps"), py::arg("tol"));
 
     py::class_<Ensemble>(m, "Ensemble", py::dynamic_attr())
         .def(py::init<const std::vector<Field>&, const std::vector<Stencil>&, const std::vector<Model>&,

@@ -15,7 +15,8 @@ times the CPU FP64 oracle (oracle/, the checker) on this host on a bounded sampl
 
 Other workloads (--workload): c1 (256^2 FP64 spinodal), c2 (1024^2 FP32 noisy
 active), c3 (default), c3f64 (4096^2 FP64), c4 (batched ensemble of 512
-realisations, sides 64..512, vs sequential plain runs on the <=256^2 subset).
+realisations, sides 64..512, vs sequential plain runs on the <=256^2 subset), c5
+(learned surrogate with Euler audits vs plain Euler, 256^2; --steps = coarse steps).
 """
 import argparse
 import json
@@ -218,12 +219,74 @@ def bench_copy(args):
                                               "bytes_per_launch": nbytes}}
 
 
+def bench_surrogate(args):
+    """C5: the learned flux-form surrogate with its Euler judge on one 256 x 256 FP32
+    field at T = 0 (lambda = 1, zeta = -2, phi0 = 0, phi = 0.01 N(0,1) from numpy seed 5),
+    Delta = 100 Euler steps of dt = 5e-3 per coarse step.  --steps coarse steps are
+    timed for the hybrid integrator (audit every --audit-every coarse steps, tolerance
+    2%) and the same span is timed for plain explicit Euler; the Euler end state is the
+    reference for the structure-factor first moment <k> and the L2 error."""
+    from activemodelbplus import surrogate as SG
+    from activemodelbplus.integrator import Integrator, Model, Stencil, Surrogate
+
+    side, delta = 256, SG.DELTA_STEPS
+    rng = np.random.default_rng(5)
+    phi = 0.01 * rng.normal(size=(side, side))
+    model, st = Model(-0.25, 0, 0.25, 1, 1.0, -2.0, 0.0), Stencil(SG.DT, 1, 1)
+    sur = Surrogate(side, side, max_batch=1, seed=0)
+    weights = SG.default_weights_path()
+    trained = os.path.exists(weights)
+    if trained:
+        SG.load_weights(sur, weights)
+    coarse = args.steps
+    # warm-up: graphs, tiles and the audit path
+    w = SG.HybridIntegrator(sur, phi, model, st, audit_every=args.audit_every)
+    w.run_coarse(min(args.warmup, 2 * args.audit_every))
+    sim = Integrator(phi, st, model, seed=0, precision=32)
+    sim.run(min(args.warmup, 2) * delta)
+    sim.set_field(phi)
+
+    h = SG.HybridIntegrator(sur, phi, model, st, audit_every=args.audit_every)
+    t0 = time.perf_counter()
+    h.run_coarse(coarse)
+    f_h = h.field
+    t_hybrid = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    sim.run(coarse * delta)
+    f_e = sim.field
+    t_euler = time.perf_counter() - t0
+    only = Integrator(phi, st, model, seed=0, precision=32)
+    sur.advance(only, 2, [1.0, -2.0, 0.0], delta)
+    only.set_field(phi)
+    dev_ms = sur.advance(only, coarse, [1.0, -2.0, 0.0], delta)
+    sites = side * side
+    k_e, k_h = SG.mean_wavenumber(f_e), SG.mean_wavenumber(f_h)
+    s = h.stats
+    return {
+        "metric": METRIC, "value": sites * delta * coarse / t_hybrid, "unit": "site-updates/s (Euler-equivalent)",
+        "n_gpus": 1, "steps": coarse, "warmup": args.warmup, "ms_per_step": t_hybrid * 1e3 / coarse,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (fp32 accumulate)",
+        "data": "synthetic (seeded numpy)" + ("" if trained else "; untrained weights (no weight file)"),
+        "config": {"workload": f"C5 surrogate {side}x{side} T=0, Delta={delta} dt, t={coarse * delta * SG.DT:g}",
+                   "audit_every": args.audit_every, "tol": 0.02},
+        "speedup_vs_euler": t_euler / t_hybrid,
+        "euler_site_updates_per_s": sites * delta * coarse / t_euler,
+        "surrogate_only_us_per_step": dev_ms * 1e3 / coarse,
+        "fallback_coarse_steps": s["fallback_steps"], "audits": s["audits"],
+        "max_audit_error": s["max_audit_error"],
+        "k_mean_euler": k_e, "k_mean_rel_err": k_h / k_e - 1,
+        "l2_rel_err_vs_euler": float(np.linalg.norm(f_h - f_e) / np.linalg.norm(f_e)),
+        "mass_drift": float(f_h.sum() - phi.astype(np.float32).sum()),
+    }
+
+
 def main():
     p = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=2000)
     p.add_argument("--warmup", type=int, default=64)
-    p.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c4", "copy"])
+    p.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c4", "c5", "copy"])
+    p.add_argument("--audit-every", type=int, default=50, help="c5: coarse steps per Euler audit")
     p.add_argument("--realisations", type=int, default=512)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     p.add_argument("--autotune", action="store_true", help="pick the tile shape by timing")
@@ -235,6 +298,8 @@ def main():
         sys.exit(2)
     if args.workload == "c4":
         out = bench_batched(args)
+    elif args.workload == "c5":
+        out = bench_surrogate(args)
     elif args.workload == "copy":
         out = bench_copy(args)
     else:

@@ -172,7 +172,8 @@ int amb_batch_autotune(amb_batch* b, int reps, int* chosen);
 
 /* A periodic U-Net (3 levels, 32/64/128 channels, 3x3 kernels, GELU; BF16 weights
  * and activations on matrix cores, FP32 accumulation) mapping (phi, lambda, zeta,
- * phi0) to a current J; one surrogate step is phi <- phi - div J with the
+ * phi0) to a current J (its FP32 head scaled by 1e-3; phi enters as bfloat16 plus its
+ * rounding remainder, ~16 bits); one surrogate step is phi <- phi - div J with the
  * backward-difference divergence, so sum(phi) is conserved exactly.  Grid sides must
  * be multiples of 64.  Parameters live in one FP32 vector (amb_surrogate_param_count
  * floats); training uses Adam with optional gradient-norm clipping. */
@@ -184,7 +185,8 @@ int amb_surrogate_param_count(const amb_surrogate* s, int64_t* n);
 int amb_surrogate_get_params(amb_surrogate* s, float* out);
 int amb_surrogate_set_params(amb_surrogate* s, const float* in);
 
-/* Loss (mean over the batch of |phi - div J - target|^2 / |target - phi|^2) and its
+/* Loss (mean over the batch of |phi - div J - target|^2 / (|target - phi|^2 + 1e-8 ny nx),
+ * the floor keeping nearly stationary samples from dominating) and its
  * gradient with respect to the parameter vector (grad nullable), for host inputs of
  * batch fields (ny*nx each) and batch parameter triples (lambda, zeta, phi0). */
 int amb_surrogate_loss_grad(amb_surrogate* s, const double* phi, const double* params, const double* target,
@@ -204,6 +206,16 @@ int amb_surrogate_predict(amb_surrogate* s, const double* phi, const double* par
  * counter advances by nsteps * euler_steps_per_step.  device_ms: HIP-event time. */
 int amb_surrogate_advance(amb_surrogate* s, amb_handle* h, int64_t nsteps, const double* params3,
                           int64_t euler_steps_per_step, double* device_ms);
+
+/* Audit of one surrogate step against the explicit-Euler path, on the device.  From
+ * the current field of `main`, `judge` (a second FP32 integrator on the same grid)
+ * runs euler_steps Euler steps and the surrogate takes one step on `main`;
+ * *rel_err = |phi_S - phi_E|_2 / |phi_E|_2 (FP64 sums).  If rel_err <= tol `main`
+ * keeps the surrogate field (*accepted = 1), otherwise, and whenever either field is
+ * not finite, it takes the Euler field (*accepted = 0).  main's step counter advances
+ * by euler_steps either way. */
+int amb_surrogate_audit(amb_surrogate* s, amb_handle* main, amb_handle* judge, const double* params3,
+                        int64_t euler_steps, double tol, double* rel_err, int* accepted);
 
 #ifdef __cplusplus
 }

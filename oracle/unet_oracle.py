@@ -9,7 +9,7 @@ accumulation order).  Its own backward pass is pinned by finite differences of i
 forward in tests/test_unet_oracle.py.
 
 Network (NHWC, periodic padding, 3x3 convolutions unless noted):
-  x0  = [phi, lamb/2, zeta/5, phi0/0.6, 0...]            16 channels
+  x0  = [phi, lamb/2, zeta/5, phi0/0.6, 256 (phi - bf16(phi)), 0...]   16 channels
   e1  = gelu(conv(x0; W1) + b1)                            32 ch, H x W
   p1  = avgpool2(e1)                                        32 ch, H/2
   e2  = gelu(conv(p1; W2) + b2)                            64 ch, H/2
@@ -21,14 +21,18 @@ Network (NHWC, periodic padding, 3x3 convolutions unless noted):
   u1  = conv1x1(d2; W6) + b6                                32 ch, H/2
   s1  = up2(u1) + e1                                        32 ch, H
   d1  = gelu(conv(s1; W7) + b7)                            32 ch, H
-  J   = conv(d1; Wh) + bh                                   2 ch (y, x), FP32
+  J   = 1e-3 (conv(d1; Wh) + bh)                            2 ch (y, x), FP32
   phi' = phi - div J,  (div J)(i,j) = Jy(i,j) - Jy(i-1,j) + Jx(i,j) - Jx(i,j-1)
-Loss (per batch): mean over samples of sum_p (phi' - target)^2 / sum_p (target - phi)^2.
+Loss (per batch): mean over samples of sum_p (phi' - target)^2 / (sum_p (target - phi)^2
++ LOSS_FLOOR * sites).
 Weights W[tap][co][ci] with tap = 3*(dy+1) + (dx+1) reading in[y+dy][x+dx].
 """
 from __future__ import annotations
 
 import numpy as np
+
+# Per-site floor of the per-sample loss normaliser |target - phi|^2 + floor * sites.
+LOSS_FLOOR = 1e-8
 
 LAYERS = [  # name, cin, cout, taps, gelu
     ("c1", 16, 32, 9, True),
@@ -41,6 +45,9 @@ LAYERS = [  # name, cin, cout, taps, gelu
     ("head", 32, 2, 9, False),
 ]
 PARAM_SCALE = (2.0, 5.0, 0.6)  # lamb, zeta, phi0 are divided by these in the input
+LO_SCALE = 256.0  # input channel 4: LO_SCALE (phi - bf16(phi))
+J_SCALE = 1e-3  # J = J_SCALE (head convolution)
+CIN_REAL = 5  # input channels carrying data; the rest of c1's 16 are zero
 C0, C1 = 0.7978845608028654, 0.044715
 
 
@@ -139,9 +146,12 @@ def unpack(theta):
 
 
 def make_input(phi, params):
+    """Input channels from the field as the device holds it (float32)."""
     n, h, w = phi.shape
+    p32 = np.asarray(phi, dtype=np.float32)
     x = np.zeros((n, h, w, 16))
-    x[..., 0] = phi
+    x[..., 0] = p32
+    x[..., 4] = (p32 - bf16(p32).astype(np.float32)) * np.float32(LO_SCALE)
     for c in range(3):
         x[..., 1 + c] = (np.asarray(params)[:, c] / PARAM_SCALE[c])[:, None, None]
     return x
@@ -171,7 +181,7 @@ def forward(theta, phi, params, emulate_bf16=True):
     c["u1"] = r(conv(c["d2"], *Wq["c6"]))
     c["s1"] = r(up2(c["u1"]) + c["e1"])
     c["d1"] = act("c7", c["s1"])
-    J = conv(c["d1"], *Wq["head"])
+    J = J_SCALE * conv(c["d1"], *Wq["head"])
     return J, phi - div_back(J), c
 
 
@@ -183,11 +193,11 @@ def loss_and_grad(theta, phi, params, target, emulate_bf16=True):
     Wq = {k: (r(w) if k != "head" else w, b) for k, (w, b) in P.items()}
     J, pred, c = forward(theta, phi, params, emulate_bf16)
     n = phi.shape[0]
-    norm = ((target - phi) ** 2).sum(axis=(1, 2))
+    norm = ((target - phi) ** 2).sum(axis=(1, 2)) + LOSS_FLOOR * phi.shape[1] * phi.shape[2]
     res = pred - target
     loss = float(np.mean((res**2).sum(axis=(1, 2)) / norm))
     g = 2 * res / (norm[:, None, None] * n)
-    dJ = -div_back_T(g)
+    dJ = -J_SCALE * div_back_T(g)
     grads = {}
     dd1, dwh, dbh = conv_grad(c["d1"], Wq["head"][0], dJ)
     grads["head"] = (dwh, dbh)
@@ -222,17 +232,17 @@ def loss_and_grad(theta, phi, params, target, emulate_bf16=True):
 
 
 def init_params(seed):
-    """He-uniform weights (zero on the 12 padding input channels of c1), zero biases."""
+    """He-uniform weights (zero on the 11 padding input channels of c1), zero biases."""
     rng = np.random.default_rng(seed)
     layout, total = param_layout()
     theta = np.zeros(total)
     for name, w_off, b_off, cin, cout, taps in layout:
-        real_cin = 4 if name == "c1" else cin
+        real_cin = CIN_REAL if name == "c1" else cin
         bound = np.sqrt(6.0 / (taps * real_cin))
         if name == "head":
             bound *= 0.1
         w = rng.uniform(-bound, bound, size=(taps, cout, cin))
         if name == "c1":
-            w[:, :, 4:] = 0
+            w[:, :, CIN_REAL:] = 0
         theta[w_off:b_off] = w.reshape(-1)
     return theta

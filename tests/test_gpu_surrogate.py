@@ -29,7 +29,7 @@ def perturbed(S, seed):
     theta = S.get_params().astype(np.float64)
     theta += 0.03 * np.random.default_rng(seed).normal(size=theta.size)
     layout, _ = U.param_layout()
-    theta[layout[0][1]:layout[0][2]].reshape(9, 32, 16)[:, :, 4:] = 0
+    theta[layout[0][1]:layout[0][2]].reshape(9, 32, 16)[:, :, 5:] = 0
     S.set_params(theta.astype(np.float32))
     return S.get_params().astype(np.float64)
 
@@ -75,9 +75,9 @@ def test_gradients_match_oracle(amb):
 def test_training_reduces_loss(amb):
     S = amb.Surrogate(64, 64, max_batch=8, seed=7)
     phi, params, target = batch(16, 64, 64, 8)
-    # a smooth, learnable target: one explicit diffusion-like step of the field
+    # a smooth, learnable target: a small explicit diffusion-like step of the field
     lap = (np.roll(phi, 1, 1) + np.roll(phi, -1, 1) + np.roll(phi, 1, 2) + np.roll(phi, -1, 2) - 4 * phi)
-    target = phi + 0.1 * lap
+    target = phi + 2e-3 * lap
     S.set_dataset(phi.astype(np.float32), target.astype(np.float32), params.astype(np.float32))
     l0 = S.loss_grad(phi[:8], params[:8], target[:8])[0]
     S.train(300, 8, 2e-3, 1.0)
@@ -105,3 +105,43 @@ def test_advance_on_integrator(amb):
             ref = S.predict(ref[None], np.array([p3]))[0][0]
         np.testing.assert_allclose(sim.field, ref, atol=1e-5)
         assert abs(sim.field.sum() - start.sum()) < 1e-3
+
+
+def test_audit_accepts_rejects_and_falls_back(amb):
+    """Device audit: within tolerance main keeps the surrogate step (bit-identical to
+    predict); with tol = 0, and with corrupted (NaN) weights at any tolerance, main
+    takes exactly the Euler field.  The step counter advances by the Euler steps."""
+    S = amb.Surrogate(64, 64, max_batch=1, seed=12)
+    perturbed(S, 13)
+    rng = np.random.default_rng(14)
+    phi0 = 0.1 * rng.normal(size=(64, 64))
+    model, stencil = amb.Model(-0.25, 0, 0.25, 1, 0.5, -1, 0), amb.Stencil(5e-3, 1, 1)
+    main = amb.Integrator(phi0, stencil, model, seed=1, precision=32)
+    judge = amb.Integrator(phi0, stencil, model, seed=1, precision=32)
+    p3 = [0.5, -1.0, 0.0]
+    start = main.field
+    euler = amb.Integrator(start, stencil, model, seed=1, precision=32)
+    euler.run(100)
+    phi_e = euler.field
+    phi_s = S.predict(start[None], np.array([p3]))[0][0]
+    want = np.linalg.norm(phi_s - phi_e) / np.linalg.norm(phi_e)
+
+    err, ok = S.audit(main, judge, p3, 100, 1e9)
+    assert ok and err == pytest.approx(want, rel=1e-6)
+    np.testing.assert_array_equal(main.field, phi_s)
+    assert main.timestep == 100
+
+    main.set_field(start)
+    err, ok = S.audit(main, judge, p3, 100, 0.0)
+    assert not ok and err == pytest.approx(want, rel=1e-6)
+    np.testing.assert_array_equal(main.field, phi_e)
+    assert main.timestep == 200
+
+    theta = S.get_params()
+    theta[-5] = np.nan  # a head weight
+    S.set_params(theta)
+    main.set_field(start)
+    err, ok = S.audit(main, judge, p3, 100, 1e9)
+    assert not ok and not np.isfinite(err)
+    np.testing.assert_array_equal(main.field, phi_e)
+    assert abs(main.field.sum() - start.sum()) < 1e-3
