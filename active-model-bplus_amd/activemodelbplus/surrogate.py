@@ -26,40 +26,11 @@ from .integrator import Ensemble, Integrator, Model, Stencil, Surrogate
 
 DT = 5e-3
 DELTA_STEPS = 100
-BIN_EDGES = np.linspace(0, 3, 300)[1:]
 
 
 # ---- observables --------------------------------------------------------------------
-
-def structure_factor(phi, dx=1.0, dy=1.0, bin_edges=BIN_EDGES):
-    """Radially binned S(k) = <|phi_k|^2> of a periodic field (the reference notebook's
-    examples/structureFactor.ipynb cell 7 convention: phi_k = FFT(phi - mean) dx dy,
-    bins ``linspace(0, 3, 300)[1:]``).  Returns (k centres, S) with NaN for empty bins."""
-    phi = np.asarray(phi, dtype=np.float64)
-    ny, nx = phi.shape
-    phiq = np.abs(np.fft.fft2(phi - phi.mean()) * dx * dy) ** 2
-    kx = 2 * np.pi * np.fft.fftfreq(nx, d=dx)
-    ky = 2 * np.pi * np.fft.fftfreq(ny, d=dy)
-    K = np.sqrt(kx[None, :] ** 2 + ky[:, None] ** 2)
-    idx = np.digitize(K, bin_edges, right=True)
-    nb = len(bin_edges) - 1
-    counts = np.bincount(idx.ravel(), minlength=nb + 2)[1:nb + 1]
-    sums = np.bincount(idx.ravel(), weights=phiq.ravel(), minlength=nb + 2)[1:nb + 1]
-    with np.errstate(invalid="ignore", divide="ignore"):
-        S = sums / counts
-    return 0.5 * (bin_edges[1:] + bin_edges[:-1]), S
-
-
-def mean_wavenumber(phi, **kw):
-    """First moment <k> = sum k S(k) / sum S(k) over the occupied bins."""
-    k, S = structure_factor(phi, **kw)
-    ok = np.isfinite(S)
-    return float(np.sum(k[ok] * S[ok]) / np.sum(S[ok]))
-
-
-def peak_wavenumber(phi, **kw):
-    k, S = structure_factor(phi, **kw)
-    return float(k[np.nanargmax(S)])
+# The structure factor follows examples/structureFactor.ipynb cell 7 (see trajectory.py).
+from .trajectory import BIN_EDGES, mean_wavenumber, peak_wavenumber, structure_factor  # noqa: E402,F401
 
 
 # ---- data -----------------------------------------------------------------------------
@@ -114,12 +85,28 @@ def generate_pairs(n_real=64, side=256, t_max=1e4, n_times=312, seed=5, log=prin
 
 # ---- training --------------------------------------------------------------------------
 
+def validation_loss(sur, phi_in, phi_out, params, batch=16):
+    """Mean per-sample loss over a held-out set (no parameter update)."""
+    total, n = 0.0, phi_in.shape[0]
+    for i in range(0, n, batch):
+        j = min(n, i + batch)
+        loss, _ = sur.loss_grad(phi_in[i:j].astype(np.float64), params[i:j].astype(np.float64),
+                                phi_out[i:j].astype(np.float64))
+        total += loss * (j - i)
+    return total / n
+
+
 def train(sur, phi_in, phi_out, params, schedule=((4000, 1e-3), (3000, 3e-4), (1000, 1e-4)), batch=16,
-          clip=1.0, chunk=200, log=print):
+          clip=1.0, chunk=200, log=print, noise=0.0, noise_seed=0, val=None, val_every=1000):
     """Adam on random mini-batches; ``schedule`` = ((iterations, learning rate), ...).
-    Returns the list of (iteration, mean loss of the chunk)."""
+    ``noise``: relative training-input noise (``Surrogate.set_train_noise``).  With a
+    held-out set ``val`` = (phi_in, phi_out, params) the validation loss is computed
+    every ``val_every`` iterations and the best parameters are kept.
+    Returns the list of (iteration, mean training loss of the chunk, validation loss)."""
     sur.set_dataset(phi_in, phi_out, params)
+    sur.set_train_noise(noise, noise_seed)
     hist, it, t0 = [], 0, time.time()
+    best = (np.inf, None)
     for n, lr in schedule:
         done = 0
         while done < n:
@@ -127,8 +114,18 @@ def train(sur, phi_in, phi_out, params, schedule=((4000, 1e-3), (3000, 3e-4), (1
             loss = sur.train(k, batch, lr, clip)
             done += k
             it += k
-            hist.append((it, loss))
-            log(f"  train: it {it:6d}  lr {lr:.1e}  loss {loss:.4e}  {time.time() - t0:6.1f} s")
+            v = None
+            if val is not None and (it % val_every == 0 or it == sum(m for m, _ in schedule)):
+                v = validation_loss(sur, *val)
+                if v < best[0]:
+                    best = (v, sur.get_params().copy())
+            hist.append((it, loss, v))
+            log(f"  train: it {it:6d}  lr {lr:.1e}  loss {loss:.4e}" + (f"  val {v:.4e}" if v is not None else "")
+                + f"  {time.time() - t0:6.1f} s")
+    sur.set_train_noise(0.0)
+    if best[1] is not None:
+        sur.set_params(best[1])
+        log(f"  kept the parameters of the best validation loss {best[0]:.4e}")
     return hist
 
 
@@ -151,25 +148,30 @@ def load_weights(sur, path):
 
 class HybridIntegrator:
     """Advances an FP32 field in coarse steps of ``delta_steps`` Euler steps with the
-    surrogate, auditing it against the Euler path every ``audit_every`` coarse steps.
+    surrogate, auditing it against the Euler path.
 
     Audit (``Surrogate.audit``, on the device): from the same state, one surrogate
     step and ``delta_steps`` Euler steps on a second integrator, the judge; error =
     |phi_S - phi_E| / |phi_E| (L2).  If the error exceeds ``tol`` (or a field is not
-    finite) the Euler result is kept and the next ``audit_every - 1`` coarse steps run
-    on the Euler path (fallback); otherwise the run continues with the surrogate.  Mass
-    is conserved exactly on both paths.
+    finite) the Euler result is kept and the coarse steps up to the next audit run on
+    the Euler path (fallback); otherwise they run on the surrogate.  The audit interval
+    starts at ``audit_every`` coarse steps, doubles after an audit with error at most
+    tol / 2, up to ``max_audit_every`` (default: fixed interval), and returns to
+    ``audit_every`` after a failed audit.  Mass is conserved exactly on both paths.
     """
 
     def __init__(self, sur, field, model, stencil=None, delta_steps=DELTA_STEPS, audit_every=20, tol=0.02,
-                 seed=0):
+                 seed=0, max_audit_every=None):
         stencil = stencil or Stencil(DT, 1, 1)
         self.sur = sur
         self.model, self.stencil = model, stencil
         self.params3 = [model.lamb, model.zeta, float(np.mean(field))]
         self.main = Integrator(field, stencil, model, seed=seed, precision=32)
         self.judge = Integrator(field, stencil, model, seed=seed, precision=32)
-        self.delta, self.audit_every, self.tol = delta_steps, audit_every, tol
+        self.delta, self.tol = delta_steps, tol
+        self.audit_min = max(1, int(audit_every))
+        self.audit_max = max(self.audit_min, int(max_audit_every or audit_every))
+        self.interval = self.audit_min
         self.stats = dict(coarse_steps=0, surrogate_steps=0, fallback_steps=0, audits=0, audit_failures=0,
                           max_audit_error=0.0, audit_errors=[])
 
@@ -189,6 +191,9 @@ class HybridIntegrator:
         st["max_audit_error"] = max(st["max_audit_error"], err) if np.isfinite(err) else float("inf")
         if not ok:
             st["audit_failures"] += 1
+            self.interval = self.audit_min
+        elif err <= 0.5 * self.tol:
+            self.interval = min(2 * self.interval, self.audit_max)
         return ok
 
     def run_coarse(self, n):
@@ -203,7 +208,7 @@ class HybridIntegrator:
                 st["surrogate_steps"] += 1
             else:
                 st["fallback_steps"] += 1
-            k = min(self.audit_every - 1, n - done)
+            k = min(self.interval - 1, n - done)
             if k <= 0:
                 continue
             if ok:
@@ -222,12 +227,14 @@ class HybridIntegrator:
 
 # ---- evaluation ------------------------------------------------------------------------
 
-def evaluate(sur, n_eval=4, side=256, t_max=1e4, seed=1234, audit_every=20, tol=0.02, checkpoints=(10, 100, 1000, 1e4),
-             log=print, euler_timing=True):
+def evaluate(sur, n_eval=4, side=256, t_max=1e4, seed=1234, audit_every=10, tol=0.02, checkpoints=(10, 100, 1000, 1e4),
+             log=print, euler_timing=True, max_audit_every=200):
     """Held-out parameter draws: Euler reference vs hybrid vs surrogate-only vs coarse Euler.
 
     Returns a dict of metrics: S(k) first moment and peak at each checkpoint time, mass
-    drift, wall times and the speedup of the hybrid over plain Euler to t_max."""
+    drift, wall times and the speedup of the hybrid over plain Euler to t_max.  A second
+    Euler realisation per member (same parameters, other initial noise) gives the
+    natural scatter of <k> against which the surrogate's error is read."""
     rng = np.random.default_rng(seed)
     params = sample_params(rng, n_eval)
     # keep held-out draws inside the spinodal region so that they phase-separate
@@ -239,16 +246,19 @@ def evaluate(sur, n_eval=4, side=256, t_max=1e4, seed=1234, audit_every=20, tol=
     res = {"params": params.tolist(), "checkpoints": cps, "euler": [], "hybrid": [], "surrogate_only": []}
 
     # Euler reference (batched for speed; each member equals a plain run bit for bit)
-    ens = Ensemble(fields, st, models, list(range(n_eval)), precision=32)
+    twins = initial_fields(np.random.default_rng(seed + 1), params, side)
+    ens = Ensemble(fields + twins, st, models + models, list(range(2 * n_eval)), precision=32)
     ens.autotune(2)
     cur = 0
-    ref_fields = {}
+    ref_fields, twin_fields = {}, {}
     for c in cps:
         target = int(round(c / DT))
         ens.run(target - cur)
         cur = target
         ref_fields[c] = [ens.field(i) for i in range(n_eval)]
+        twin_fields[c] = [ens.field(n_eval + i) for i in range(n_eval)]
         log(f"  euler reference at t={c:g}")
+    res["euler_twin_k_mean"] = [{str(c): mean_wavenumber(twin_fields[c][i]) for c in cps} for i in range(n_eval)]
     for i in range(n_eval):
         res["euler"].append({str(c): dict(k_mean=mean_wavenumber(ref_fields[c][i]),
                                          k_peak=peak_wavenumber(ref_fields[c][i]),
@@ -258,8 +268,11 @@ def evaluate(sur, n_eval=4, side=256, t_max=1e4, seed=1234, audit_every=20, tol=
     t_hybrid = []
     for i in range(n_eval):
         for mode in ("hybrid", "surrogate_only"):
-            h = HybridIntegrator(sur, fields[i], models[i], st, audit_every=audit_every if mode == "hybrid" else 10**9,
-                                 tol=tol if mode == "hybrid" else 1e9)
+            if mode == "hybrid":
+                h = HybridIntegrator(sur, fields[i], models[i], st, audit_every=audit_every, tol=tol,
+                                     max_audit_every=max_audit_every)
+            else:
+                h = HybridIntegrator(sur, fields[i], models[i], st, audit_every=10**9, tol=1e9)
             m0 = float(h.field.sum())
             out, prev, t0 = {}, 0.0, time.perf_counter()
             for c in cps:
@@ -305,6 +318,10 @@ def evaluate(sur, n_eval=4, side=256, t_max=1e4, seed=1234, audit_every=20, tol=
     errs_s = [res["surrogate_only"][i]["metrics"][last]["k_mean"] / res["euler"][i][last]["k_mean"] - 1
               for i in range(n_eval)]
     res["k_mean_rel_err_surrogate_only"] = errs_s
+    res["k_mean_rel_diff_euler_twin"] = [res["euler_twin_k_mean"][i][last] / res["euler"][i][last]["k_mean"] - 1
+                                         for i in range(n_eval)]
+    res["k_peak_rel_err_hybrid"] = [res["hybrid"][i]["metrics"][last]["k_peak"] / res["euler"][i][last]["k_peak"] - 1
+                                    for i in range(n_eval)]
     return res
 
 
@@ -319,11 +336,14 @@ def _main(argv=None):
     t.add_argument("--iterations", type=int, default=8000)
     t.add_argument("--batch", type=int, default=16)
     t.add_argument("--seed", type=int, default=5)
+    t.add_argument("--noise", type=float, default=0.3, help="relative training-input noise")
+    t.add_argument("--val-realisations", type=int, default=8)
     e = sub.add_parser("evaluate")
     e.add_argument("--weights", required=True)
     e.add_argument("--members", type=int, default=4)
     e.add_argument("--t-max", type=float, default=1e4)
-    e.add_argument("--audit-every", type=int, default=20)
+    e.add_argument("--audit-every", type=int, default=10)
+    e.add_argument("--max-audit-every", type=int, default=200)
     e.add_argument("--tol", type=float, default=0.02)
     e.add_argument("--out")
     a = p.parse_args(argv)
@@ -335,19 +355,28 @@ def _main(argv=None):
         t0 = time.time()
         data = generate_pairs(a.realisations, 256, a.t_max, a.times, a.seed, log=log)
         log(f"dataset: {data[0].shape[0]} pairs in {time.time() - t0:.1f} s")
+        val = None
+        if a.val_realisations > 0:
+            v = generate_pairs(a.val_realisations, 256, a.t_max, a.times // 4, a.seed + 1000, log=log)
+            val = v[:3]
+            log(f"validation: {v[0].shape[0]} pairs")
         sur = Surrogate(256, 256, max_batch=a.batch, seed=a.seed)
         it = a.iterations
         sched = ((int(it * 0.5), 1e-3), (int(it * 0.375), 3e-4), (it - int(it * 0.5) - int(it * 0.375), 1e-4))
         t1 = time.time()
-        hist = train(sur, *data[:3], schedule=sched, batch=a.batch, log=log)
+        hist = train(sur, *data[:3], schedule=sched, batch=a.batch, log=log, noise=a.noise, noise_seed=a.seed,
+                     val=val)
+        vals = [h[2] for h in hist if h[2] is not None]
         meta = dict(pairs=int(data[0].shape[0]), realisations=a.realisations, t_max=a.t_max, iterations=it,
-                    batch=a.batch, train_s=time.time() - t1, final_loss=hist[-1][1], delta_steps=DELTA_STEPS, dt=DT)
+                    batch=a.batch, train_s=time.time() - t1, final_loss=hist[-1][1], delta_steps=DELTA_STEPS, dt=DT,
+                    noise=a.noise, best_val_loss=min(vals) if vals else None)
         save_weights(sur, a.out, meta)
         log(json.dumps(meta))
     else:
         sur = Surrogate(256, 256, max_batch=1, seed=0)
         meta = load_weights(sur, a.weights)
-        res = evaluate(sur, a.members, 256, a.t_max, audit_every=a.audit_every, tol=a.tol, log=log)
+        res = evaluate(sur, a.members, 256, a.t_max, audit_every=a.audit_every, tol=a.tol, log=log,
+                       max_audit_every=a.max_audit_every)
         res["weights_meta"] = meta
         text = json.dumps(res)
         if a.out:

@@ -23,7 +23,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIBAMB = os.path.join(LIBDIR, "libamb.so")
 PKG = os.path.join(HERE, "activemodelbplus", "integrator")
 
-HIP_SOURCES = ["amb_step.hip", "amb_capi.hip", "amb_batch.hip", "amb_unet.hip"]
+HIP_SOURCES = ["amb_step.hip", "amb_capi.hip", "amb_batch.hip", "amb_unet.hip", "amb_sk.hip"]
 ARCH = os.environ.get("AMB_OFFLOAD_ARCH", "gfx950")
 
 # Packed FP32 VALU ops are disabled for all device code.
@@ -65,11 +65,11 @@ def build_lib(verbose: bool = False) -> str:
         _run([_hipcc(), *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj], quiet=not verbose)
         return obj
 
-    with ThreadPoolExecutor(max_workers=min(4, len(HIP_SOURCES))) as ex:
+    with ThreadPoolExecutor(max_workers=min(5, len(HIP_SOURCES))) as ex:
         objs = list(ex.map(compile_one, HIP_SOURCES))
     tmp = LIBAMB + ".tmp"
     _run([_hipcc(), "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs,
-          "-Wl,-rpath,/opt/rocm/lib"])
+          "-L/opt/rocm/lib", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIBAMB)
     return LIBAMB
 

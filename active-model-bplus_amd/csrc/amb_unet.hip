@@ -28,6 +28,7 @@
 
 #include "amb.h"
 #include "amb_internal.h"
+#include "amb_math.h"
 
 namespace amb {
 namespace unet {
@@ -227,40 +228,58 @@ __global__ __launch_bounds__(256) void conv_kernel(const ConvIn src, int cin, co
         }
 }
 
-// Inference convolution (3x3, periodic) on a TY x 16 pixel tile per block of four
-// waves.  The block first stages its (TY + 2) x 18 input pixels (halo included) in
-// LDS, applying the operand transform of SRC (input assembly, 2x2 average pooling or
-// up-sampling add) once per pixel; the 3x3 taps then read A fragments from LDS.  The
-// waves form WM row groups x WN channel groups: each accumulates TY / WM row tiles of
-// 16 pixels against 16 NB / WN output channels over all K = 9 CIN, with the weight
-// fragments of up to five K-steps ahead in flight.  Output is
-// GELU(conv + bias) in bfloat16; NB2 > 0 instead applies a fused 1x1 convolution to
-// 16 NB2 channels (no activation) to the bfloat16 GELU tile kept in LDS.
-template <int SRC, int CIN, int NB, int TY, int WM, int WN, int NB2 = 0>
-__global__ __launch_bounds__(256) void conv_tile_kernel(const ConvIn src, const bf16* __restrict__ w,
-                                                        const float* __restrict__ bias, int H, int W,
-                                                        bf16* __restrict__ out, const bf16* __restrict__ w2,
-                                                        const float* __restrict__ b2)
+// Inference convolution (3x3, periodic) on a TY x 16 pixel tile per block.  The block
+// first stages its (TY + 2) x 18 input pixels (halo included) in LDS, applying the
+// operand transform of SRC (input assembly, 2x2 average pooling or up-sampling add)
+// once per pixel; the 3x3 taps then read A fragments from LDS.  The block's waves form
+// WM row groups x WN channel groups x KW K groups: each accumulates TY / WM row tiles
+// of 16 pixels against 16 NB / WN output channels over every KW-th 32-wide K-step of
+// K = 9 CIN, with weight fragments of up to five of its K-steps ahead in flight; K
+// groups are summed through LDS in a fixed order.  Output is GELU(conv + bias) in
+// bfloat16; NB2 > 0 instead applies a fused 1x1 convolution to 16 NB2 channels (no
+// activation) to the bfloat16 GELU tile kept in LDS.  Small tiles give many blocks
+// per CU, whose memory latencies then overlap.
+template <int SRC, int CIN, int NB, int TY, int WM, int WN, int NB2 = 0, int KW = 1>
+__global__ __launch_bounds__(64 * WM * WN * KW) void conv_tile_kernel(const ConvIn src, const bf16* __restrict__ w,
+                                                                      const float* __restrict__ bias, int H, int W,
+                                                                      bf16* __restrict__ out,
+                                                                      const bf16* __restrict__ w2,
+                                                                      const float* __restrict__ b2)
 {
     constexpr int COUT = 16 * NB, LDP = CIN + 8, PW = 18, K = 9 * CIN, NS = (K + 31) / 32;
-    constexpr int MB = TY / WM, NBW = NB / WN, TLD = COUT + 8;
-    static_assert(WM * WN == 4 && TY % WM == 0 && NB % WN == 0 && CIN % 8 == 0, "tile shape");
+    constexpr int MB = TY / WM, NBW = NB / WN, TLD = COUT + 8, NT = 64 * WM * WN * KW, NWAVE = NT / 64;
+    constexpr int SPW = (NS + KW - 1) / KW;  // K-steps of the wave with kw = 0
+    static_assert(TY % WM == 0 && NB % WN == 0 && CIN % 8 == 0 && NT <= 1024, "tile shape");
     static_assert(NB2 == 0 || COUT % 32 == 0, "fused 1x1 needs COUT % 32 == 0");
     __shared__ __align__(16) bf16 sin[(TY + 2) * PW * LDP];
     __shared__ __align__(16) bf16 tl[NB2 > 0 ? TY * 16 * TLD : 8];
+    __shared__ float red[KW > 1 ? KW * TY * 16 * COUT : 1];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int q = lane >> 4, l16 = lane & 15;
-    const int wm = wave % WM, wn = wave / WM;
+    const int wm = wave % WM, wn = (wave / WM) % WN, kw = wave / (WM * WN);
     const int tiles_x = W / 16, tiles_y = H / TY;
     const int bx = blockIdx.x % tiles_x, t = blockIdx.x / tiles_x, by = t % tiles_y, n = t / tiles_y;
     const int x0 = 16 * bx, y0 = TY * by;
 
-    // stage the padded input tile
-    for (int e = threadIdx.x; e < (TY + 2) * PW * (CIN / 8); e += 256) {
-        const int c8 = e % (CIN / 8), pp = e / (CIN / 8);
-        const int ry = pp / PW, rx = pp - ry * PW;
-        const bf16x8 v = load_a<SRC>(src, CIN, n, wrapi(y0 + ry - 1, H), wrapi(x0 + rx - 1, W), H, W, 8 * c8);
-        *reinterpret_cast<bf16x8*>(sin + pp * LDP + 8 * c8) = v;
+    // stage the padded input tile: every gather of the thread first, then the stores
+    constexpr int ITEMS = (TY + 2) * PW * (CIN / 8), PER = (ITEMS + NT - 1) / NT;
+    bf16x8 sv[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int e = threadIdx.x + NT * k;
+        if (e < ITEMS) {
+            const int c8 = e % (CIN / 8), pp = e / (CIN / 8);
+            const int ry = pp / PW, rx = pp - ry * PW;
+            sv[k] = load_a<SRC>(src, CIN, n, wrapi(y0 + ry - 1, H), wrapi(x0 + rx - 1, W), H, W, 8 * c8);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int e = threadIdx.x + NT * k;
+        if (e < ITEMS) {
+            const int c8 = e % (CIN / 8), pp = e / (CIN / 8);
+            *reinterpret_cast<bf16x8*>(sin + pp * LDP + 8 * c8) = sv[k];
+        }
     }
 
     const bf16x8 zero = {};
@@ -268,7 +287,7 @@ __global__ __launch_bounds__(256) void conv_tile_kernel(const ConvIn src, const 
         const int k0 = 32 * st + 8 * q;
 #pragma unroll
         for (int j = 0; j < NBW; ++j) bb[j] = zero;
-        if (k0 < K) {
+        if (st < NS && k0 < K) {
             const int tap = k0 / CIN, ci = k0 - tap * CIN;
 #pragma unroll
             for (int j = 0; j < NBW; ++j) bb[j] = load8(w + ((size_t)tap * COUT + 16 * (wn * NBW + j) + l16) * CIN + ci);
@@ -279,61 +298,80 @@ __global__ __launch_bounds__(256) void conv_tile_kernel(const ConvIn src, const 
     for (int m = 0; m < MB; ++m)
 #pragma unroll
         for (int j = 0; j < NBW; ++j) acc[m][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    // ring of PF K-steps of weight fragments in flight
-    constexpr int PF = NS < 6 ? NS : 6;
+    // ring of PF of the wave's K-steps of weight fragments in flight
+    constexpr int PF = SPW < 6 ? SPW : 6;
     bf16x8 bq[PF][NBW];
 #pragma unroll
-    for (int i = 0; i < PF - 1; ++i) load_b(i, bq[i]);
+    for (int i = 0; i < PF - 1; ++i) load_b(kw + KW * i, bq[i]);
     __syncthreads();
 #pragma unroll
-    for (int st = 0; st < NS; ++st) {
-        if (st + PF - 1 < NS) load_b(st + PF - 1, bq[(st + PF - 1) % PF]);
+    for (int jj = 0; jj < SPW; ++jj) {
+        if (jj + PF - 1 < SPW) load_b(kw + KW * (jj + PF - 1), bq[(jj + PF - 1) % PF]);
+        const int st = kw + KW * jj;
         const int k0 = 32 * st + 8 * q;
         const int tap = k0 / CIN, ci = k0 - tap * CIN;
         const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+        const bool live = st < NS && k0 < K;
 #pragma unroll
         for (int m = 0; m < MB; ++m) {
             const int r = wm + WM * m;
             bf16x8 a = zero;
-            if (k0 < K) a = *reinterpret_cast<const bf16x8*>(sin + ((r + 1 + dy) * PW + l16 + 1 + dx) * LDP + ci);
+            if (live) a = *reinterpret_cast<const bf16x8*>(sin + ((r + 1 + dy) * PW + l16 + 1 + dx) * LDP + ci);
 #pragma unroll
             for (int j = 0; j < NBW; ++j)
-                acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bq[st % PF][j], acc[m][j], 0, 0, 0);
+                acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bq[jj % PF][j], acc[m][j], 0, 0, 0);
         }
     }
 
-    // epilogue: D fragment element r of lane (q, l16) is pixel 4q + r, channel l16
+    // epilogue: D fragment element e of lane (q, l16) is pixel 4q + e, channel l16
+    auto emit = [&](int r, int px, int co, float v) {
+        const bf16 g = f2bf(gelu(v + bias[co]));
+        if constexpr (NB2 > 0) {
+            tl[(r * 16 + px) * TLD + co] = g;
+        } else {
+            out[(((size_t)n * H + y0 + r) * W + x0 + px) * COUT + co] = g;
+        }
+    };
+    if constexpr (KW == 1) {
 #pragma unroll
-    for (int m = 0; m < MB; ++m) {
-        const int r = wm + WM * m;
+        for (int m = 0; m < MB; ++m)
 #pragma unroll
-        for (int j = 0; j < NBW; ++j) {
-            const int co = 16 * (wn * NBW + j) + l16;
-            const float bco = bias[co];
+            for (int j = 0; j < NBW; ++j)
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const bf16 g = f2bf(gelu(acc[m][j][e] + bco));
-                if constexpr (NB2 > 0) {
-                    tl[(r * 16 + 4 * q + e) * TLD + co] = g;
-                } else {
-                    out[(((size_t)n * H + y0 + r) * W + x0 + 4 * q + e) * COUT + co] = g;
-                }
-            }
+                for (int e = 0; e < 4; ++e) emit(wm + WM * m, 4 * q + e, 16 * (wn * NBW + j) + l16, acc[m][j][e]);
+    } else {
+#pragma unroll
+        for (int m = 0; m < MB; ++m)
+#pragma unroll
+            for (int j = 0; j < NBW; ++j)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    red[(kw * TY * 16 + (wm + WM * m) * 16 + 4 * q + e) * COUT + 16 * (wn * NBW + j) + l16] =
+                        acc[m][j][e];
+        __syncthreads();
+        for (int i = threadIdx.x; i < TY * 16 * COUT; i += NT) {
+            float v = red[i];
+#pragma unroll
+            for (int k = 1; k < KW; ++k) v += red[k * TY * 16 * COUT + i];
+            const int co = i % COUT, pix = i / COUT;
+            emit(pix / 16, pix % 16, co, v);
         }
     }
     if constexpr (NB2 > 0) {
         __syncthreads();
         constexpr int C2 = 16 * NB2;
         // TY row tiles x NB2 channel blocks of the 1x1 output, spread over the waves
-        for (int job = wave; job < TY * NB2; job += 4) {
+        for (int job = wave; job < TY * NB2; job += NWAVE) {
             const int r = job / NB2, j = job % NB2;
-            f32x4 d = {0.f, 0.f, 0.f, 0.f};
+            bf16x8 a2[COUT / 32], bw[COUT / 32];
 #pragma unroll
             for (int ks = 0; ks < COUT / 32; ++ks) {
-                const bf16x8 a = *reinterpret_cast<const bf16x8*>(tl + (r * 16 + l16) * TLD + 32 * ks + 8 * q);
-                const bf16x8 b = load8(w2 + (size_t)(16 * j + l16) * COUT + 32 * ks + 8 * q);
-                d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, d, 0, 0, 0);
+                a2[ks] = *reinterpret_cast<const bf16x8*>(tl + (r * 16 + l16) * TLD + 32 * ks + 8 * q);
+                bw[ks] = load8(w2 + (size_t)(16 * j + l16) * COUT + 32 * ks + 8 * q);
             }
+            f32x4 d = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < COUT / 32; ++ks) d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[ks], bw[ks], d, 0, 0, 0);
             const float bb = b2[16 * j + l16];
 #pragma unroll
             for (int e = 0; e < 4; ++e)
@@ -517,7 +555,7 @@ __global__ void head_fwd_kernel(const bf16* __restrict__ d1, const float* __rest
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npix) return;
     const int x = p % W, t = p / W, y = t % H, n = t / H;
-    // one accumulator per tap column, summed as in head_flux_kernel (same rounding)
+    // one accumulator per tap column, summed as in c7_head_flux_kernel (same rounding)
     float j0[3] = {bh[0], 0.f, 0.f}, j1[3] = {bh[1], 0.f, 0.f};
 #pragma unroll
     for (int tap = 0; tap < 9; ++tap) {
@@ -535,37 +573,116 @@ __global__ void head_fwd_kernel(const bf16* __restrict__ d1, const float* __rest
     J[2 * (size_t)p + 1] = kJScale * ((j1[0] + j1[1]) + j1[2]);
 }
 
-// Inference: J on a 16x16 tile plus the row above and the column to its left (from a
-// 19x19 patch of d1 in LDS), then phi_out = phi - div J for the tile.
-__global__ __launch_bounds__(320) void head_flux_kernel(const bf16* __restrict__ d1, const float* __restrict__ wh,
-                                                        const float* __restrict__ bh, const float* __restrict__ phi,
-                                                        int H, int W, float* __restrict__ phi_out)
+// Inference tail in one kernel per 16 x 16 tile: c7 (3x3, 32 -> 32, GELU) on the
+// up-sampled u1 plus the skip e1, the head (3x3, 32 -> 2, FP32 weights) and the flux
+// update phi_out = phi - div J.  The block stages s1 = up(u1) + e1 over rows
+// y0-3 .. y0+17 and cols x0-3 .. x0+17 in LDS, computes d1 on rows y0-2 .. y0+16 and
+// cols x0-2 .. x0+16 with MFMA (two overlapping 16-wide column tiles per row) into
+// LDS, J on rows y0-1 .. y0+15 and cols x0-1 .. x0+15, then the update of its 16 x 16
+// pixels.  J_out (nullable) receives the tile's J.  Eight waves; 64 KB of LDS.
+__global__ __launch_bounds__(512) void c7_head_flux_kernel(const bf16* __restrict__ u1, const bf16* __restrict__ e1,
+                                                           const bf16* __restrict__ w7, const float* __restrict__ b7,
+                                                           const float* __restrict__ wh, const float* __restrict__ bh,
+                                                           const float* __restrict__ phi, int H, int W,
+                                                           float* __restrict__ phi_out, float* __restrict__ J_out)
 {
-    constexpr int SP = 40;  // LDS pixel stride (32 channels + 8 padding)
-    __shared__ __align__(16) bf16 sd[19 * 19 * SP];
-    __shared__ float sj[17 * 17 * 2];
+    constexpr int SP = 40;                   // LDS pixel stride (32 channels + 8 padding)
+    constexpr int SW = 21, DW = 19, JW = 17;  // staged, d1 and J region widths
+    constexpr int NS = 9;                    // K-steps of c7 (K = 9 x 32)
+    constexpr int ROWT = 2 * DW, MBW = (ROWT + 7) / 8;  // 38 row tiles over 8 waves
+    __shared__ __align__(16) bf16 sin[SW * SW * SP];   // later reused for J
+    __shared__ __align__(16) bf16 sd[DW * DW * SP];
+    float* sj = reinterpret_cast<float*>(sin);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int q = lane >> 4, l16 = lane & 15;
     const int tilesx = W / 16, tilesy = H / 16;
     const int b = blockIdx.x, tx = b % tilesx, t2 = b / tilesx, ty = t2 % tilesy, n = t2 / tilesy;
     const int x0 = tx * 16, y0 = ty * 16;
-    // d1 patch rows y0-2 .. y0+16, cols x0-2 .. x0+16, 32 channels (4 x 16-byte chunks)
-    for (int i = threadIdx.x; i < 19 * 19 * 4; i += 320) {
-        const int chunk = i & 3, pix = i >> 2, r = pix / 19, c = pix % 19;
-        const int yy = wrapi(y0 - 2 + r, H), xx = wrapi(x0 - 2 + c, W);
-        *reinterpret_cast<bf16x8*>(sd + pix * SP + 8 * chunk) = load8(d1 + (((size_t)n * H + yy) * W + xx) * 32 + 8 * chunk);
+
+    // stage s1 = up(u1) + e1: every gather of the thread first, then the stores
+    constexpr int ITEMS = SW * SW * 4, PER = (ITEMS + 511) / 512;
+    bf16x8 sv[PER];
+    ConvIn src{u1, e1, nullptr, nullptr};
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int i = threadIdx.x + 512 * k;
+        if (i < ITEMS) {
+            const int c4 = i & 3, pix = i >> 2, r = pix / SW, c = pix % SW;
+            sv[k] = load_a<kSrcUpAdd>(src, 32, n, wrapi(y0 - 3 + r, H), wrapi(x0 - 3 + c, W), H, W, 8 * c4);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int i = threadIdx.x + 512 * k;
+        if (i < ITEMS) *reinterpret_cast<bf16x8*>(sin + (i >> 2) * SP + 8 * (i & 3)) = sv[k];
+    }
+
+    // c7: row tile rt = (row i in 0..18, column tile ct): d1 row y0-2+i, columns
+    // x0-2+16ct' .. with ct' = 0 at x0-2 and ct' = 1 at x0+1 (overlapping)
+    const bf16x8 zero = {};
+    f32x4 acc[MBW][2];
+#pragma unroll
+    for (int m = 0; m < MBW; ++m) acc[m][0] = acc[m][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+    auto load_b = [&](int st, bf16x8* bb) {
+        const int k0 = 32 * st + 8 * q, tap = k0 / 32, ci = k0 - 32 * tap;
+        bb[0] = load8(w7 + ((size_t)tap * 32 + l16) * 32 + ci);
+        bb[1] = load8(w7 + ((size_t)tap * 32 + 16 + l16) * 32 + ci);
+    };
+    constexpr int PF = 6;
+    bf16x8 bq[PF][2];
+#pragma unroll
+    for (int i = 0; i < PF - 1; ++i) load_b(i, bq[i]);
+    __syncthreads();
+#pragma unroll
+    for (int st = 0; st < NS; ++st) {
+        if (st + PF - 1 < NS) load_b(st + PF - 1, bq[(st + PF - 1) % PF]);
+        const int k0 = 32 * st + 8 * q, tap = k0 / 32, ci = k0 - 32 * tap;
+        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+#pragma unroll
+        for (int m = 0; m < MBW; ++m) {
+            const int rt = wave + 8 * m;
+            if (rt < ROWT) {
+                const int i = rt >> 1, c0 = (rt & 1) ? 3 : 0;  // d1 row, first column (d1 coordinates)
+                // staged coordinates are d1 coordinates + 1
+                const bf16x8 a =
+                    *reinterpret_cast<const bf16x8*>(sin + ((i + 1 + dy) * SW + c0 + l16 + 1 + dx) * SP + ci);
+                acc[m][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bq[st % PF][0], acc[m][0], 0, 0, 0);
+                acc[m][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bq[st % PF][1], acc[m][1], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < MBW; ++m) {
+        const int rt = wave + 8 * m;
+        if (rt < ROWT) {
+            const int i = rt >> 1, c0 = (rt & 1) ? 3 : 0;
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int co = 16 * j + l16;
+                const float bco = b7[co];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int px = 4 * q + e, col = c0 + px;
+                    // the second column tile writes only the columns the first lacks
+                    if (c0 == 0 || px >= 13) sd[(i * DW + col) * SP + co] = f2bf(gelu(acc[m][j][e] + bco));
+                }
+            }
+        }
     }
     __syncthreads();
-    // J at rows y0-1 .. y0+15, cols x0-1 .. x0+15 (patch coordinates 1 .. 17); the
-    // weights are uniform across the block and read as scalars
-    if (threadIdx.x < 17 * 17) {
-        const int i = threadIdx.x, r = i / 17 + 1, c = i % 17 + 1;
+
+    // head: J at d1 coordinates 1 .. 17 (rows y0-1 .. y0+15); the weights are uniform
+    // across the block and read as scalars; summation order as in head_fwd_kernel
+    if (threadIdx.x < JW * JW) {
+        const int i = threadIdx.x, r = i / JW + 1, c = i % JW + 1;
         float j0[3] = {bh[0], 0.f, 0.f}, j1[3] = {bh[1], 0.f, 0.f};
 #pragma unroll
         for (int tap = 0; tap < 9; ++tap) {
-            const bf16* src = sd + ((r + tap / 3 - 1) * 19 + (c + tap % 3 - 1)) * SP;
+            const bf16* sp = sd + ((r + tap / 3 - 1) * DW + (c + tap % 3 - 1)) * SP;
             const float* w0 = wh + tap * 64;
 #pragma unroll
             for (int c8 = 0; c8 < 4; ++c8) {
-                const u32x4 v = *reinterpret_cast<const u32x4*>(src + 8 * c8);
+                const u32x4 v = *reinterpret_cast<const u32x4*>(sp + 8 * c8);
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const int ci = 8 * c8 + 2 * e;
@@ -583,10 +700,14 @@ __global__ __launch_bounds__(320) void head_flux_kernel(const bf16* __restrict__
     __syncthreads();
     if (threadIdx.x < 256) {
         const int r = threadIdx.x / 16, c = threadIdx.x % 16;  // tile pixel
-        const int i = (r + 1) * 17 + (c + 1), up = r * 17 + (c + 1), left = (r + 1) * 17 + c;
+        const int i = (r + 1) * JW + (c + 1), up = r * JW + (c + 1), left = (r + 1) * JW + c;
         const float div = (sj[2 * i] - sj[2 * up]) + (sj[2 * i + 1] - sj[2 * left + 1]);
         const size_t p = ((size_t)n * H + y0 + r) * W + x0 + c;
         phi_out[p] = phi[p] - div;
+        if (J_out) {
+            J_out[2 * p] = sj[2 * i];
+            J_out[2 * p + 1] = sj[2 * i + 1];
+        }
     }
 }
 
@@ -724,6 +845,40 @@ __global__ void dj_kernel(const float* __restrict__ g, int H, int W, int npix, f
     dJ[2 * (size_t)p + 1] = kJScale * (g[((size_t)n * H + y) * W + wrapi(x + 1, W)] - gp);
 }
 
+// Training-input noise: phi += div_back(eta), eta ~ N(0, sig_n^2) per pixel and
+// component with sig_n = rel * sqrt(norm_n / hw) / 2 (norm_n: the sample's loss
+// normaliser), so the perturbation is a divergence (sum(phi) unchanged) of about rel
+// times the sample's RMS increment.  eta comes from Philox4x32-10 with counter
+// {pixel, iteration low, iteration high, 0x6e6f6973} and key seed, Box-Muller.
+__device__ __forceinline__ float2 train_eta(uint32_t pix, uint64_t iter, uint32_t k0, uint32_t k1)
+{
+    U4 c;
+    c.x = pix;
+    c.y = (uint32_t)iter;
+    c.z = (uint32_t)(iter >> 32);
+    c.w = 0x6e6f6973u;
+    const U4 r = philox4x32_10(c, k0, k1);
+    const float u1 = ((float)r.x + 0.5f) * 2.3283064e-10f, u2 = (float)r.y * 2.3283064e-10f;
+    const float m = sqrtf(-2.f * logf(u1));
+    float sn, cs;
+    sincosf(6.2831853f * u2, &sn, &cs);
+    return make_float2(m * cs, m * sn);
+}
+
+__global__ void train_noise_kernel(float* __restrict__ phi, const float* __restrict__ norm, int H, int W, int npix,
+                                   float rel, uint64_t iter, uint32_t k0, uint32_t k1)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    const int x = p % W, t = p / W, y = t % H, n = t / H;
+    const int hw = H * W;
+    const float sig = 0.5f * rel * sqrtf(norm[n] / (float)hw);
+    const int up = (n * H + wrapi(y - 1, H)) * W + x, left = (n * H + y) * W + wrapi(x - 1, W);
+    const float2 e = train_eta((uint32_t)p, iter, k0, k1);
+    const float ey_up = train_eta((uint32_t)up, iter, k0, k1).x, ex_left = train_eta((uint32_t)left, iter, k0, k1).y;
+    phi[p] += sig * ((e.x - ey_up) + (e.y - ex_left));
+}
+
 // ---- optimiser -------------------------------------------------------------------------
 
 __global__ void sumsq_kernel(const float* __restrict__ g, int n, float* __restrict__ part)
@@ -773,6 +928,16 @@ __global__ void pack_kernel(const float* __restrict__ master, int taps, int cout
     const bf16 v = f2bf(master[i]);
     wf[i] = v;
     wb[((size_t)tap * cin + ci) * cout + co] = v;
+}
+
+// c1 inference weights: the first 8 of the 16 input channels (the data channels and
+// zero pads), [tap][co][8].
+__global__ void pack_c1_kernel(const float* __restrict__ master, bf16* __restrict__ w8)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 9 * 32 * 8) return;
+    const int ci = i % 8, t = i / 8;  // t = tap * 32 + co
+    w8[i] = f2bf(master[(size_t)t * kCin0 + ci]);
 }
 
 __global__ void gather_kernel(const float* __restrict__ src, const int* __restrict__ idx, int per, int count,
@@ -832,6 +997,7 @@ struct amb_surrogate {
     float* adam_v = nullptr;
     int64_t adam_t = 0;
     bf16* wf[7] = {};
+    bf16* wf0_8 = nullptr;  // c1 weights [tap][co][8] over the first 8 input channels (inference)
     bf16* wb[7] = {};
     // activations (bf16 NHWC) and gradients
     bf16 *x0 = nullptr, *e1 = nullptr, *e1p = nullptr, *p1 = nullptr, *e2 = nullptr, *e2p = nullptr, *p2 = nullptr;
@@ -850,6 +1016,8 @@ struct amb_surrogate {
     float *ds_in = nullptr, *ds_out = nullptr, *ds_par = nullptr, *ds_norm = nullptr;
     int64_t ds_count = 0;
     std::mt19937_64 rng;
+    float train_noise = 0.f;  // relative size of the training-input noise (0: off)
+    uint64_t noise_seed = 0, noise_iter = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipGraphExec_t step_graph = nullptr;  // one surrogate step on a field, batch 1
     void* graph_field = nullptr;
@@ -863,8 +1031,9 @@ std::vector<void*> all_buffers(amb_surrogate* s)
     std::vector<void*> v = {s->master, s->grad, s->adam_m, s->adam_v, s->x0, s->e1, s->e1p, s->p1, s->e2, s->e2p,
                             s->p2, s->bt, s->btp, s->u2, s->s2, s->d2, s->d2p, s->u1, s->s1, s->d1, s->d1p, s->dpre,
                             s->J, s->phi, s->target, s->params, s->norm, s->pred, s->g, s->dJ, s->dd1, s->ds1,
-                            s->dd2, s->ds2, s->dbt, s->dp2, s->dp1, s->part, s->part2, s->audit_part, s->p3, s->loss_part, s->sumsq_part, s->d_idx,
-                            s->ds_in, s->ds_out, s->ds_par, s->ds_norm};
+                            s->dd2, s->ds2, s->dbt, s->dp2, s->dp1, s->part, s->part2, s->audit_part, s->p3,
+                            s->loss_part, s->sumsq_part, s->d_idx,
+                            s->ds_in, s->ds_out, s->ds_par, s->ds_norm, s->wf0_8};
     for (int l = 0; l < 7; ++l) {
         v.push_back(s->wf[l]);
         v.push_back(s->wb[l]);
@@ -894,13 +1063,13 @@ hipError_t conv_launch(const ConvIn& in, int cin, const bf16* w, const float* bi
     return hipGetLastError();
 }
 
-template <int SRC, int CIN, int NB, int TY, int WM, int WN, int NB2 = 0>
+template <int SRC, int CIN, int NB, int TY, int WM, int WN, int NB2 = 0, int KW = 1>
 hipError_t conv_tile_launch(const ConvIn& in, const bf16* w, const float* bias, int n, int H, int W, bf16* out,
                             hipStream_t st, const bf16* w2 = nullptr, const float* b2 = nullptr)
 {
     if (H % TY || W % 16) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((conv_tile_kernel<SRC, CIN, NB, TY, WM, WN, NB2>), dim3(n * (H / TY) * (W / 16)), dim3(256), 0,
-                       st, in, w, bias, H, W, out, w2, b2);
+    hipLaunchKernelGGL((conv_tile_kernel<SRC, CIN, NB, TY, WM, WN, NB2, KW>), dim3(n * (H / TY) * (W / 16)),
+                       dim3(64 * WM * WN * KW), 0, st, in, w, bias, H, W, out, w2, b2);
     return hipGetLastError();
 }
 
@@ -996,7 +1165,10 @@ int pack_weights(amb_surrogate* s)
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return fail_hip(e, "weight pack");
     }
-    return AMB_OK;
+    hipLaunchKernelGGL(pack_c1_kernel, dim3(blocks_for(9 * 32 * 8)), dim3(kEw), 0, s->stream, s->master + s->w_off[0],
+                       s->wf0_8);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? AMB_OK : fail_hip(e, "weight pack");
 }
 
 #define SUR_CHECK(expr)                                   \
@@ -1036,30 +1208,31 @@ int forward(amb_surrogate* s, int n, const float* phi, const float* params, floa
     return AMB_OK;
 }
 
-// Inference: phi_out = phi - div J in six launches, all split-K: pooling, up-sampling
-// adds and the input layer are fused into the convolutions' operand loads, the 1x1
-// layers into the epilogue of the 3x3 layer before them, and the head into the flux
-// update.  The head input d1 stays in memory (predict reads J from it).
-int forward_infer(amb_surrogate* s, int n, const float* phi, const float* params, float* phi_out)
+// Inference: phi_out = phi - div J in five launches: pooling, up-sampling adds and the
+// input layer are fused into the convolutions' LDS staging, the 1x1 layers into the
+// epilogue of the 3x3 layer before them, and c7 with the head and the flux update.
+// J_out (nullable) receives the current.
+int forward_infer(amb_surrogate* s, int n, const float* phi, const float* params, float* phi_out,
+                  float* J_out = nullptr)
 {
     const int H = s->H, W = s->W, H2 = H / 2, W2 = W / 2, H4 = H / 4, W4 = W / 4;
     const int P1 = n * H * W, P2 = n * H2 * W2, P3 = n * H4 * W4;
     hipStream_t st = s->stream;
     const float* M = s->master;
-    // <SRC, CIN, NB, TY, WM, WN, NB2>
-    SUR_CHECK((conv_tile_launch<kSrcInput, kCin0, 2, 8, 4, 1>(ConvIn{nullptr, nullptr, phi, params}, s->wf[0],
-                                                               M + s->b_off[0], n, H, W, s->e1, st)));
-    SUR_CHECK((conv_tile_launch<kSrcPool, 32, 4, 4, 4, 1>(plain(s->e1), s->wf[1], M + s->b_off[1], n, H2, W2, s->e2,
+    // <SRC, CIN, NB, TY, WM, WN, NB2>: small tiles, so that several blocks share a CU
+    // and hide each other's memory latency
+    static_assert(kCinReal <= 8, "c1 inference packs the data channels into 8");
+    SUR_CHECK((conv_tile_launch<kSrcInput, 8, 2, 4, 2, 2>(ConvIn{nullptr, nullptr, phi, params}, s->wf0_8,
+                                                           M + s->b_off[0], n, H, W, s->e1, st)));
+    SUR_CHECK((conv_tile_launch<kSrcPool, 32, 4, 1, 1, 4>(plain(s->e1), s->wf[1], M + s->b_off[1], n, H2, W2, s->e2,
                                                            st)));
-    SUR_CHECK((conv_tile_launch<kSrcPool, 64, 8, 2, 2, 2, 4>(plain(s->e2), s->wf[2], M + s->b_off[2], n, H4, W4, s->u2,
-                                                              st, s->wf[3], M + s->b_off[3])));
-    SUR_CHECK((conv_tile_launch<kSrcUpAdd, 64, 4, 4, 4, 1, 2>(ConvIn{s->u2, s->e2, nullptr, nullptr}, s->wf[4],
+    SUR_CHECK((conv_tile_launch<kSrcPool, 64, 8, 1, 1, 4, 4, 2>(plain(s->e2), s->wf[2], M + s->b_off[2], n, H4, W4,
+                                                                 s->u2, st, s->wf[3], M + s->b_off[3])));
+    SUR_CHECK((conv_tile_launch<kSrcUpAdd, 64, 4, 2, 1, 4, 2>(ConvIn{s->u2, s->e2, nullptr, nullptr}, s->wf[4],
                                                                M + s->b_off[4], n, H2, W2, s->u1, st, s->wf[5],
                                                                M + s->b_off[5])));
-    SUR_CHECK((conv_tile_launch<kSrcUpAdd, 32, 2, 8, 4, 1>(ConvIn{s->u1, s->e1, nullptr, nullptr}, s->wf[6],
-                                                            M + s->b_off[6], n, H, W, s->d1, st)));
-    hipLaunchKernelGGL(head_flux_kernel, dim3(n * (H / 16) * (W / 16)), dim3(320), 0, st, s->d1, M + s->w_off[7],
-                       M + s->b_off[7], phi, H, W, phi_out);
+    hipLaunchKernelGGL(c7_head_flux_kernel, dim3(n * (H / 16) * (W / 16)), dim3(512), 0, st, s->u1, s->e1, s->wf[6],
+                       M + s->b_off[6], M + s->w_off[7], M + s->b_off[7], phi, H, W, phi_out, J_out);
     SUR_CHECK(hipGetLastError());
     return AMB_OK;
 }
@@ -1239,6 +1412,7 @@ int amb_surrogate_create(int64_t ny, int64_t nx, int max_batch, uint64_t seed, a
         const size_t n = (size_t)kLayers[l].taps * kLayers[l].cout * kLayers[l].cin;
         A(&s->wf[l], n * 2);
         A(&s->wb[l], n * 2);
+        if (l == 0) A(&s->wf0_8, 9 * 32 * 8 * 2);
     }
     A(&s->x0, P1 * kCin0 * 2);
     A(&s->e1, P1 * 32 * 2);
@@ -1432,6 +1606,11 @@ int amb_surrogate_train(amb_surrogate* s, int64_t iterations, int batch, double 
         hipLaunchKernelGGL(gather_kernel, dim3(blocks_for(3 * batch)), dim3(kEw), 0, s->stream, s->ds_par, s->d_idx, 3,
                            batch, s->params);
         hipLaunchKernelGGL(gather_kernel, dim3(1), dim3(kEw), 0, s->stream, s->ds_norm, s->d_idx, 1, batch, s->norm);
+        if (s->train_noise > 0.f) {
+            hipLaunchKernelGGL(train_noise_kernel, dim3(blocks_for((size_t)hw * batch)), dim3(kEw), 0, s->stream, s->phi,
+                               s->norm, s->H, s->W, hw * batch, s->train_noise, s->noise_iter++,
+                               (uint32_t)s->noise_seed, (uint32_t)(s->noise_seed >> 32));
+        }
         SUR_CHECK(hipGetLastError());
         double loss = 0;
         int rc = loss_and_grad(s, batch, mean_loss ? &loss : nullptr);
@@ -1444,6 +1623,17 @@ int amb_surrogate_train(amb_surrogate* s, int64_t iterations, int batch, double 
     return AMB_OK;
 }
 
+int amb_surrogate_set_train_noise(amb_surrogate* s, double rel, uint64_t seed)
+{
+    clear_error();
+    if (!s) return fail(AMB_E_ARG, "surrogate must not be null");
+    if (!(rel >= 0.0) || rel > 10.0) return fail(AMB_E_ARG, "noise must be in [0, 10]");
+    s->train_noise = (float)rel;
+    s->noise_seed = seed;
+    s->noise_iter = 0;
+    return AMB_OK;
+}
+
 // One surrogate step on host fields: phi_next = phi - div J (batch fields of ny*nx).
 int amb_surrogate_predict(amb_surrogate* s, const double* phi, const double* params, int batch, double* phi_next,
                           double* current)
@@ -1453,14 +1643,9 @@ int amb_surrogate_predict(amb_surrogate* s, const double* phi, const double* par
     if (batch < 1 || batch > s->max_batch) return fail(AMB_E_ARG, "batch out of range");
     // the inference kernels of advance(), so both give identical fields
     int rc = upload_batch(s, batch, phi, params, nullptr);
-    if (rc == AMB_OK) rc = forward_infer(s, batch, s->phi, s->params, s->pred);
+    if (rc == AMB_OK) rc = forward_infer(s, batch, s->phi, s->params, s->pred, current ? s->J : nullptr);
     if (rc != AMB_OK) return rc;
     const int P1 = batch * s->H * s->W;
-    if (current) {
-        hipLaunchKernelGGL(head_fwd_kernel, dim3(blocks_for(P1)), dim3(kEw), 0, s->stream, s->d1, s->master + s->w_off[7],
-                           s->master + s->b_off[7], s->H, s->W, P1, s->J);
-        SUR_CHECK(hipGetLastError());
-    }
     std::vector<float> out(P1), jj;
     SUR_CHECK(hipMemcpyAsync(out.data(), s->pred, (size_t)P1 * 4, hipMemcpyDeviceToHost, s->stream));
     if (current) {

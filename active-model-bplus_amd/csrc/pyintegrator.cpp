@@ -169,6 +169,21 @@ class Integrator {
         return ms;
     }
 
+    py::tuple structure_factor_bins(py::array_t<double, py::array::c_style | py::array::forcecast> edges)
+    {
+        if (edges.ndim() != 1 || edges.shape(0) < 1) throw py::value_error("edges must be a non-empty 1-D array");
+        const int ne = (int)edges.shape(0);
+        py::array_t<double> sums(ne + 1);
+        py::array_t<int64_t> counts(ne + 1);
+        int rc;
+        {
+            py::gil_scoped_release release;
+            rc = amb_structure_factor(h_, edges.data(), ne, sums.mutable_data(), counts.mutable_data());
+        }
+        check(rc);
+        return py::make_tuple(sums, counts);
+    }
+
     py::tuple mass() const
     {
         double s = 0;
@@ -280,6 +295,8 @@ class Surrogate {
             throw py::value_error("dataset must be (count, ny, nx) x2 and (count, 3)");
         check(amb_surrogate_set_dataset(s_, phi_in.data(), phi_out.data(), params.data(), phi_in.shape(0)));
     }
+
+    void set_train_noise(double rel, uint64_t seed) { check(amb_surrogate_set_train_noise(s_, rel, seed)); }
 
     double train(int64_t iterations, int batch, double lr, double clip)
     {
@@ -527,7 +544,9 @@ PYBIND11_MODULE(integrator, m)
         .def("set_tile", &Integrator::set_tile, py::arg("index"))
         .def("autotune", &Integrator::autotune, py::arg("reps") = 8,
              "time every compiled tile shape on a copy of the state and keep the fastest")
-        .def("set_field", &Integrator::set_field, py::arg("field"), "replace the field (step counter unchanged)");
+        .def("set_field", &Integrator::set_field, py::arg("field"), "replace the field (step counter unchanged)")
+        .def("structure_factor_bins", &Integrator::structure_factor_bins, py::arg("edges"),
+             "(sums, counts) of |phi_k|^2 per numpy.digitize(|k|, edges, right=True) index, on the device");
 
     py::class_<Surrogate>(m, "Surrogate")
         .def(py::init<int64_t, int64_t, int, uint64_t>(), py::arg("ny"), py::arg("nx"), py::arg("max_batch") = 16,
@@ -538,6 +557,7 @@ PYBIND11_MODULE(integrator, m)
         .def("loss_grad", &Surrogate::loss_grad, py::arg("phi"), py::arg("params"), py::arg("target"))
         .def("set_dataset", &Surrogate::set_dataset, py::arg("phi_in"), py::arg("phi_out"), py::arg("params"))
         .def("train", &Surrogate::train, py::arg("iterations"), py::arg("batch"), py::arg("lr"), py::arg("clip") = 0.0)
+        .def("set_train_noise", &Surrogate::set_train_noise, py::arg("rel"), py::arg("seed") = 0)
         .def("predict", &Surrogate::predict, py::arg("phi"), py::arg("params"))
         .def("advance", &Surrogate::advance, py::arg("integrator"), py::arg("nsteps"), py::arg("params3"),
              py::arg("euler_steps_per_step"))

@@ -224,8 +224,10 @@ def bench_surrogate(args):
     field at T = 0 (lambda = 1, zeta = -2, phi0 = 0, phi = 0.01 N(0,1) from numpy seed 5),
     Delta = 100 Euler steps of dt = 5e-3 per coarse step.  --steps coarse steps are
     timed for the hybrid integrator (audit every --audit-every coarse steps, tolerance
-    2%) and the same span is timed for plain explicit Euler; the Euler end state is the
-    reference for the structure-factor first moment <k> and the L2 error."""
+    2%; the interval doubles after audits within half the tolerance, up to
+    --max-audit-every, and resets on a failure) and the same span is timed for plain
+    explicit Euler; the Euler end state is the reference for the structure-factor
+    first moment <k> and the L2 error."""
     from activemodelbplus import surrogate as SG
     from activemodelbplus.integrator import Integrator, Model, Stencil, Surrogate
 
@@ -240,13 +242,13 @@ def bench_surrogate(args):
         SG.load_weights(sur, weights)
     coarse = args.steps
     # warm-up: graphs, tiles and the audit path
-    w = SG.HybridIntegrator(sur, phi, model, st, audit_every=args.audit_every)
+    w = SG.HybridIntegrator(sur, phi, model, st, audit_every=args.audit_every, max_audit_every=args.max_audit_every)
     w.run_coarse(min(args.warmup, 2 * args.audit_every))
     sim = Integrator(phi, st, model, seed=0, precision=32)
     sim.run(min(args.warmup, 2) * delta)
     sim.set_field(phi)
 
-    h = SG.HybridIntegrator(sur, phi, model, st, audit_every=args.audit_every)
+    h = SG.HybridIntegrator(sur, phi, model, st, audit_every=args.audit_every, max_audit_every=args.max_audit_every)
     t0 = time.perf_counter()
     h.run_coarse(coarse)
     f_h = h.field
@@ -268,7 +270,7 @@ def bench_surrogate(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (fp32 accumulate)",
         "data": "synthetic (seeded numpy)" + ("" if trained else "; untrained weights (no weight file)"),
         "config": {"workload": f"C5 surrogate {side}x{side} T=0, Delta={delta} dt, t={coarse * delta * SG.DT:g}",
-                   "audit_every": args.audit_every, "tol": 0.02},
+                   "audit_every": [args.audit_every, args.max_audit_every], "tol": 0.02},
         "speedup_vs_euler": t_euler / t_hybrid,
         "euler_site_updates_per_s": sites * delta * coarse / t_euler,
         "surrogate_only_us_per_step": dev_ms * 1e3 / coarse,
@@ -286,7 +288,8 @@ def main():
     p.add_argument("--steps", type=int, default=2000)
     p.add_argument("--warmup", type=int, default=64)
     p.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c4", "c5", "copy"])
-    p.add_argument("--audit-every", type=int, default=50, help="c5: coarse steps per Euler audit")
+    p.add_argument("--audit-every", type=int, default=10, help="c5: initial coarse steps per Euler audit")
+    p.add_argument("--max-audit-every", type=int, default=200, help="c5: longest audit interval")
     p.add_argument("--realisations", type=int, default=512)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     p.add_argument("--autotune", action="store_true", help="pick the tile shape by timing")

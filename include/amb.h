@@ -168,6 +168,16 @@ int amb_batch_get_tiles(amb_batch* b, int* tile_per_realisation);
  * `chosen` (nullable) receives the tile index per realisation. */
 int amb_batch_autotune(amb_batch* b, int reps, int* chosen);
 
+/* ---- structure factor ------------------------------------------------------ */
+
+/* Radially binned structure factor of the handle's current field, on the device;
+ * replaces the host analysis of examples/structureFactor.ipynb (cell 7):
+ * phi_k = FFT(phi - mean) dx dy, |k| = 2 pi sqrt(fx^2 + fy^2) on numpy's fftfreq
+ * grid.  For each digitize index i = #{edges < |k|} (numpy.digitize(|k|, edges,
+ * right=True)), i = 0 .. nedges: sums[i] = sum of |phi_k|^2 (FP64) and counts[i] =
+ * the number of modes.  edges increasing, 1 <= nedges <= 1024. */
+int amb_structure_factor(amb_handle* h, const double* edges, int nedges, double* sums, int64_t* counts);
+
 /* ---- learned flux-form surrogate (no reference counterpart) --------------- */
 
 /* A periodic U-Net (3 levels, 32/64/128 channels, 3x3 kernels, GELU; BF16 weights
@@ -197,6 +207,13 @@ int amb_surrogate_loss_grad(amb_surrogate* s, const double* phi, const double* p
 int amb_surrogate_set_dataset(amb_surrogate* s, const float* phi_in, const float* phi_out, const float* params,
                               int64_t count);
 int amb_surrogate_train(amb_surrogate* s, int64_t iterations, int batch, double lr, double clip, double* mean_loss);
+
+/* Training-input noise for amb_surrogate_train: each sample's input field is
+ * perturbed by the backward-difference divergence of Gaussian noise (so its sum is
+ * unchanged) of about rel times the sample's RMS increment, drawn from a
+ * counter-based stream keyed by seed; the target is unchanged, so the network
+ * learns to damp its own errors.  rel = 0 switches it off (the default). */
+int amb_surrogate_set_train_noise(amb_surrogate* s, double rel, uint64_t seed);
 
 /* One surrogate step on host fields (current: nullable, 2*ny*nx per field [y, x]). */
 int amb_surrogate_predict(amb_surrogate* s, const double* phi, const double* params, int batch, double* phi_next,

@@ -1,0 +1,81 @@
+"""The trajectory format and the structure factor of activemodelbplus.trajectory
+against the reference's examples/structureFactor.ipynb: cell 1's file layout and
+cell 7's binning (a literal loop over bins, as the notebook writes it), a single
+Fourier mode (all its power in one bin, with the closed-form amplitude), and
+Parseval's identity over all digitize indices."""
+import numpy as np
+import pytest
+
+from activemodelbplus import trajectory as T
+from activemodelbplus.integrator import Model, Stencil
+
+
+def notebook_S(phi, dx, dy, bin_edges):
+    """Cell 7's loop: S[l] = average of |phi_q|^2 over indices == l."""
+    ny, nx = phi.shape
+    phiq = np.abs(np.fft.fftshift(np.fft.fft2(phi - np.average(phi))) * dx * dy) ** 2
+    kx = 2 * np.pi * np.fft.fftshift(np.fft.fftfreq(nx, d=dx))
+    ky = 2 * np.pi * np.fft.fftshift(np.fft.fftfreq(ny, d=dy))
+    Kx, Ky = np.meshgrid(kx, ky, indexing="xy")
+    indices = np.digitize((Kx**2 + Ky**2) ** 0.5, bin_edges, right=True)
+    k = 0.5 * (bin_edges[1:] + bin_edges[:-1])
+    S = np.zeros_like(k)
+    for l in range(S.size):
+        sel = phiq[indices == l]
+        S[l] = sel.mean() if sel.size else np.nan
+    return k, S
+
+
+@pytest.mark.parametrize("shape,dx,dy", [((64, 64), 1.0, 1.0), ((48, 80), 0.75, 1.25), ((33, 17), 1.0, 0.5)])
+def test_binning_matches_notebook(shape, dx, dy):
+    phi = np.random.default_rng(1).normal(size=shape)
+    k, S = T.structure_factor(phi, dx=dx, dy=dy)
+    k_nb, S_nb = notebook_S(phi, dx, dy, T.BIN_EDGES)
+    np.testing.assert_array_equal(k, k_nb)
+    np.testing.assert_array_equal(np.isnan(S), np.isnan(S_nb))
+    ok = ~np.isnan(S)
+    np.testing.assert_allclose(S[ok], S_nb[ok], rtol=1e-12)
+
+
+def test_single_mode():
+    n, m = 128, 5
+    x = np.arange(n)
+    phi = np.cos(2 * np.pi * m * x / n)[None, :] * np.ones((n, 1))
+    sums, counts = T.structure_factor_bins(phi)
+    k0 = 2 * np.pi * m / n
+    i = int(np.digitize([k0], T.BIN_EDGES, right=True)[0])
+    assert sums[i] == pytest.approx(n**4 / 2, rel=1e-12)  # two modes of |n^2 / 2|^2
+    others = np.delete(sums, i)
+    assert np.max(others) < 1e-12 * n**4
+
+
+def test_parseval_over_all_bins():
+    phi = np.random.default_rng(2).normal(size=(40, 56))
+    dx, dy = 0.5, 2.0
+    sums, counts = T.structure_factor_bins(phi, dx=dx, dy=dy)
+    assert counts.sum() == phi.size
+    want = phi.size * np.sum((phi - phi.mean()) ** 2) * (dx * dy) ** 2
+    assert sums.sum() == pytest.approx(want, rel=1e-12)
+
+
+def test_mean_wavenumber_of_a_ring():
+    """A field made of modes on one shell |k| = k0 has <k> within a bin of k0."""
+    n = 256
+    y, x = np.mgrid[0:n, 0:n]
+    phi = np.zeros((n, n))
+    for mx, my in [(12, 0), (0, 12), (9, 8), (8, 9)]:
+        phi += np.cos(2 * np.pi * (mx * x + my * y) / n)
+    k0 = 2 * np.pi * 12 / n
+    assert abs(T.peak_wavenumber(phi) - k0) < 0.02
+    assert abs(T.mean_wavenumber(phi) - k0) < 0.02
+    assert T.domain_size(phi) == pytest.approx(2 * np.pi / T.mean_wavenumber(phi))
+
+
+def test_trajectory_file_layout(tmp_path):
+    fields = np.random.default_rng(3).normal(size=(3, 8, 8))
+    path = tmp_path / "trajectory_1.npz"
+    T.save_trajectory(path, [1.0, 10.0, 100.0], fields, Model(-0.25, 0, 0.25, 1, 0.5, -1, 0), Stencil(1e-2, 1, 1))
+    z = T.load_trajectory(path)
+    assert set(z) == {"time", "field", "a", "b", "c", "kappa", "lamb", "zeta", "temperature", "dt", "dx", "dy"}
+    np.testing.assert_array_equal(z["field"], fields)
+    assert float(z["lamb"]) == 0.5 and float(z["dt"]) == 1e-2
