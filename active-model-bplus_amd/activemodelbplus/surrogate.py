@@ -150,8 +150,9 @@ class HybridIntegrator:
     """Advances an FP32 field in coarse steps of ``delta_steps`` Euler steps with the
     surrogate, auditing it against the Euler path.
 
-    Audit (``Surrogate.audit``, on the device): from the same state, one surrogate
-    step and ``delta_steps`` Euler steps on a second integrator, the judge; error =
+    Audit (``Surrogate.audit_begin`` / ``audit_end``, on the device): from the same
+    state, one surrogate step and ``delta_steps`` Euler steps on a second integrator,
+    the judge, which runs on its own stream while the surrogate carries on; error =
     |phi_S - phi_E| / |phi_E| (L2).  If the error exceeds ``tol`` (or a field is not
     finite) the Euler result is kept and the coarse steps up to the next audit run on
     the Euler path (fallback); otherwise they run on the surrogate.  The audit interval
@@ -172,8 +173,12 @@ class HybridIntegrator:
         self.audit_min = max(1, int(audit_every))
         self.audit_max = max(self.audit_min, int(max_audit_every or audit_every))
         self.interval = self.audit_min
-        self.stats = dict(coarse_steps=0, surrogate_steps=0, fallback_steps=0, audits=0, audit_failures=0,
-                          max_audit_error=0.0, audit_errors=[])
+        # surrogate steps taken while the judge runs, before the audit is settled: enough
+        # to hide the judge's delta_steps Euler steps, few enough that a failed audit
+        # wastes little
+        self.speculate = 16
+        self.stats = dict(coarse_steps=0, surrogate_steps=0, fallback_steps=0, discarded_steps=0, audits=0,
+                          audit_failures=0, max_audit_error=0.0, audit_errors=[])
 
     @property
     def field(self):
@@ -183,8 +188,7 @@ class HybridIntegrator:
     def time(self):
         return self.main.time
 
-    def _audit(self):
-        err, ok = self.sur.audit(self.main, self.judge, self.params3, self.delta, self.tol)
+    def _record(self, err, ok):
         st = self.stats
         st["audits"] += 1
         st["audit_errors"].append(err)
@@ -194,29 +198,36 @@ class HybridIntegrator:
             self.interval = self.audit_min
         elif err <= 0.5 * self.tol:
             self.interval = min(2 * self.interval, self.audit_max)
-        return ok
 
     def run_coarse(self, n):
-        """n coarse steps (n * delta_steps Euler-equivalent steps)."""
+        """n coarse steps (n * delta_steps Euler-equivalent steps).  Each segment of
+        ``interval`` coarse steps starts with an audit whose Euler judge runs while the
+        surrogate continues (``Surrogate.audit_begin``).  If the audit fails, the
+        segment's speculative surrogate steps are discarded (``audit_end`` restores
+        the Euler field of the audited step), the interval returns to its minimum and
+        the rest of that shorter segment runs on the Euler path.  At most
+        ``speculate`` steps run before the audit is settled; the rest of a passing
+        segment follows it."""
         st = self.stats
         done = 0
         while done < n:
-            ok = self._audit()
-            done += 1
-            st["coarse_steps"] += 1
-            if ok:
-                st["surrogate_steps"] += 1
-            else:
-                st["fallback_steps"] += 1
-            k = min(self.interval - 1, n - done)
-            if k <= 0:
-                continue
-            if ok:
-                self.sur.advance(self.main, k, self.params3, self.delta)
-                st["surrogate_steps"] += k
-            else:
-                self.main.run(k * self.delta)
+            k = min(self.interval, n - done)
+            self.sur.audit_begin(self.main, self.judge, self.params3, self.delta)
+            spec = min(k - 1, self.speculate)
+            if spec > 0:
+                self.sur.advance(self.main, spec, self.params3, self.delta)
+            err, ok = self.sur.audit_end(self.tol)
+            self._record(err, ok)
+            if not ok:
+                st["discarded_steps"] += spec
+                k = min(self.audit_min, k)
+                if k > 1:
+                    self.main.run((k - 1) * self.delta)
                 st["fallback_steps"] += k
+            else:
+                if k - 1 - spec > 0:
+                    self.sur.advance(self.main, k - 1 - spec, self.params3, self.delta)
+                st["surrogate_steps"] += k
             st["coarse_steps"] += k
             done += k
 
@@ -227,7 +238,7 @@ class HybridIntegrator:
 
 # ---- evaluation ------------------------------------------------------------------------
 
-def evaluate(sur, n_eval=4, side=256, t_max=1e4, seed=1234, audit_every=10, tol=0.02, checkpoints=(10, 100, 1000, 1e4),
+def evaluate(sur, n_eval=8, side=256, t_max=1e4, seed=1234, audit_every=10, tol=0.02, checkpoints=(10, 100, 1000, 1e4),
              log=print, euler_timing=True, max_audit_every=200):
     """Held-out parameter draws: Euler reference vs hybrid vs surrogate-only vs coarse Euler.
 
@@ -274,15 +285,16 @@ def evaluate(sur, n_eval=4, side=256, t_max=1e4, seed=1234, audit_every=10, tol=
             else:
                 h = HybridIntegrator(sur, fields[i], models[i], st, audit_every=10**9, tol=1e9)
             m0 = float(h.field.sum())
-            out, prev, t0 = {}, 0.0, time.perf_counter()
+            out, prev, wall = {}, 0.0, 0.0
             for c in cps:
+                t0 = time.perf_counter()
                 h.run_for_time(c - prev)
-                prev = c
                 f = h.field
+                wall += time.perf_counter() - t0  # rollout only, not the analysis below
+                prev = c
                 out[str(c)] = dict(k_mean=mean_wavenumber(f), k_peak=peak_wavenumber(f),
                                    mass_drift=float(f.sum() - m0), l2_vs_euler=float(
                                        np.linalg.norm(f - ref_fields[c][i]) / np.linalg.norm(ref_fields[c][i])))
-            wall = time.perf_counter() - t0
             stats = {k: v for k, v in h.stats.items() if k != "audit_errors"}
             stats["median_audit_error"] = float(np.median(h.stats["audit_errors"])) if h.stats["audit_errors"] else None
             res[mode].append(dict(metrics=out, wall_s=wall, stats=stats))
@@ -340,7 +352,7 @@ def _main(argv=None):
     t.add_argument("--val-realisations", type=int, default=8)
     e = sub.add_parser("evaluate")
     e.add_argument("--weights", required=True)
-    e.add_argument("--members", type=int, default=4)
+    e.add_argument("--members", type=int, default=8)
     e.add_argument("--t-max", type=float, default=1e4)
     e.add_argument("--audit-every", type=int, default=10)
     e.add_argument("--max-audit-every", type=int, default=200)

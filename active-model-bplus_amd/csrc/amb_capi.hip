@@ -379,6 +379,26 @@ int handle_commit(amb_handle* h, int64_t steps, bool swapped)
     return e == hipSuccess ? AMB_OK : fail_hip(e, "advance launch");
 }
 
+int handle_enqueue(amb_handle* h, int64_t steps)
+{
+    const int rc = enqueue_steps(h, steps);
+    if (rc == AMB_OK) h->timestep += steps;
+    return rc;
+}
+
+int handle_set_timestep(amb_handle* h, int64_t timestep)
+{
+    const long long v = timestep;
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(h->d_step, &v, sizeof v, hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return fail_hip(e, "set timestep");
+    h->timestep = timestep;
+    return AMB_OK;
+}
+
+int64_t handle_timestep(const amb_handle* h) { return h->timestep; }
+
 }  // namespace amb
 
 extern "C" {

@@ -34,5 +34,11 @@ int handle_view(amb_handle* h, FieldView* v);
 // Advances the step counter by `steps` (host and device); with `swapped` the field
 // now lives in the buffer that was `other`.
 int handle_commit(amb_handle* h, int64_t steps, bool swapped);
+// Enqueues `steps` explicit-Euler steps on the handle's stream without waiting (no
+// finite check) and advances the step counter.
+int handle_enqueue(amb_handle* h, int64_t steps);
+// Sets the step counter (host and device), after the handle's stream is idle.
+int handle_set_timestep(amb_handle* h, int64_t timestep);
+int64_t handle_timestep(const amb_handle* h);
 
 }  // namespace amb

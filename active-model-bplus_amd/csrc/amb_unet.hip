@@ -60,14 +60,23 @@ __host__ __device__ __forceinline__ bf16 f2bf(float f)
     return (bf16)(u >> 16);
 }
 
+// tanh(u) = 1 - 2 / (exp(2u) + 1) with the hardware 2^x (v_exp_f32) and reciprocal
+// (v_rcp_f32): saturates to +-1 for large |u|, absolute error ~1e-7 near 0, far below
+// the bfloat16 rounding the activations get.
+__device__ __forceinline__ float fast_tanh(float u)
+{
+    const float e = __builtin_amdgcn_exp2f(2.8853900817779268f * u);  // 2^(2 u log2 e)
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+}
+
 __device__ __forceinline__ float gelu(float x)
 {
-    return 0.5f * x * (1.0f + tanhf(kGeluC0 * (x + kGeluC1 * x * x * x)));
+    return 0.5f * x * (1.0f + fast_tanh(kGeluC0 * (x + kGeluC1 * x * x * x)));
 }
 
 __device__ __forceinline__ float gelu_grad(float x)
 {
-    const float t = tanhf(kGeluC0 * (x + kGeluC1 * x * x * x));
+    const float t = fast_tanh(kGeluC0 * (x + kGeluC1 * x * x * x));
     return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * kGeluC0 * (1.0f + 3.0f * kGeluC1 * x * x);
 }
 
@@ -573,34 +582,37 @@ __global__ void head_fwd_kernel(const bf16* __restrict__ d1, const float* __rest
     J[2 * (size_t)p + 1] = kJScale * ((j1[0] + j1[1]) + j1[2]);
 }
 
-// Inference tail in one kernel per 16 x 16 tile: c7 (3x3, 32 -> 32, GELU) on the
+// Inference tail in one kernel per TYO x 16 tile: c7 (3x3, 32 -> 32, GELU) on the
 // up-sampled u1 plus the skip e1, the head (3x3, 32 -> 2, FP32 weights) and the flux
 // update phi_out = phi - div J.  The block stages s1 = up(u1) + e1 over rows
-// y0-3 .. y0+17 and cols x0-3 .. x0+17 in LDS, computes d1 on rows y0-2 .. y0+16 and
-// cols x0-2 .. x0+16 with MFMA (two overlapping 16-wide column tiles per row) into
-// LDS, J on rows y0-1 .. y0+15 and cols x0-1 .. x0+15, then the update of its 16 x 16
-// pixels.  J_out (nullable) receives the tile's J.  Eight waves; 64 KB of LDS.
+// y0-3 .. y0+TYO+2 and cols x0-3 .. x0+17 in LDS, computes d1 on rows y0-2 .. y0+TYO
+// and cols x0-2 .. x0+16 with MFMA (two overlapping 16-wide column tiles per row) into
+// LDS, J on rows y0-1 .. y0+TYO-1 and cols x0-1 .. x0+15, then the update of its
+// TYO x 16 pixels.  J_out (nullable) receives the tile's J.  Eight waves.
+template <int TYO>
 __global__ __launch_bounds__(512) void c7_head_flux_kernel(const bf16* __restrict__ u1, const bf16* __restrict__ e1,
                                                            const bf16* __restrict__ w7, const float* __restrict__ b7,
                                                            const float* __restrict__ wh, const float* __restrict__ bh,
                                                            const float* __restrict__ phi, int H, int W,
                                                            float* __restrict__ phi_out, float* __restrict__ J_out)
 {
-    constexpr int SP = 40;                   // LDS pixel stride (32 channels + 8 padding)
-    constexpr int SW = 21, DW = 19, JW = 17;  // staged, d1 and J region widths
-    constexpr int NS = 9;                    // K-steps of c7 (K = 9 x 32)
-    constexpr int ROWT = 2 * DW, MBW = (ROWT + 7) / 8;  // 38 row tiles over 8 waves
-    __shared__ __align__(16) bf16 sin[SW * SW * SP];   // later reused for J
-    __shared__ __align__(16) bf16 sd[DW * DW * SP];
+    constexpr int SP = 40;                                 // LDS pixel stride (32 channels + 8 padding)
+    constexpr int SW = 21, DW = 19, JW = 17;                // staged, d1 and J region widths
+    constexpr int SH = TYO + 6, DH = TYO + 3, JH = TYO + 1;  // their heights
+    constexpr int NS = 9;                                  // K-steps of c7 (K = 9 x 32)
+    constexpr int ROWT = 2 * DH, MBW = (ROWT + 7) / 8;      // row tiles over 8 waves
+    static_assert(JH * JW * 2 * 4 <= SH * SW * SP * 2, "J must fit in the staging buffer");
+    __shared__ __align__(16) bf16 sin[SH * SW * SP];  // later reused for J
+    __shared__ __align__(16) bf16 sd[DH * DW * SP];
     float* sj = reinterpret_cast<float*>(sin);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int q = lane >> 4, l16 = lane & 15;
-    const int tilesx = W / 16, tilesy = H / 16;
+    const int tilesx = W / 16, tilesy = H / TYO;
     const int b = blockIdx.x, tx = b % tilesx, t2 = b / tilesx, ty = t2 % tilesy, n = t2 / tilesy;
-    const int x0 = tx * 16, y0 = ty * 16;
+    const int x0 = tx * 16, y0 = ty * TYO;
 
     // stage s1 = up(u1) + e1: every gather of the thread first, then the stores
-    constexpr int ITEMS = SW * SW * 4, PER = (ITEMS + 511) / 512;
+    constexpr int ITEMS = SH * SW * 4, PER = (ITEMS + 511) / 512;
     bf16x8 sv[PER];
     ConvIn src{u1, e1, nullptr, nullptr};
 #pragma unroll
@@ -617,7 +629,7 @@ __global__ __launch_bounds__(512) void c7_head_flux_kernel(const bf16* __restric
         if (i < ITEMS) *reinterpret_cast<bf16x8*>(sin + (i >> 2) * SP + 8 * (i & 3)) = sv[k];
     }
 
-    // c7: row tile rt = (row i in 0..18, column tile ct): d1 row y0-2+i, columns
+    // c7: row tile rt = (row i in 0..DH-1, column tile ct): d1 row y0-2+i, columns
     // x0-2+16ct' .. with ct' = 0 at x0-2 and ct' = 1 at x0+1 (overlapping)
     const bf16x8 zero = {};
     f32x4 acc[MBW][2];
@@ -671,9 +683,10 @@ __global__ __launch_bounds__(512) void c7_head_flux_kernel(const bf16* __restric
     }
     __syncthreads();
 
-    // head: J at d1 coordinates 1 .. 17 (rows y0-1 .. y0+15); the weights are uniform
-    // across the block and read as scalars; summation order as in head_fwd_kernel
-    if (threadIdx.x < JW * JW) {
+    // head: J at d1 coordinates 1 .. JH (rows y0-1 .. y0+TYO-1), cols 1 .. 17; the
+    // weights are uniform across the block and read as scalars; summation order as in
+    // head_fwd_kernel
+    if (threadIdx.x < JH * JW) {
         const int i = threadIdx.x, r = i / JW + 1, c = i % JW + 1;
         float j0[3] = {bh[0], 0.f, 0.f}, j1[3] = {bh[1], 0.f, 0.f};
 #pragma unroll
@@ -698,7 +711,7 @@ __global__ __launch_bounds__(512) void c7_head_flux_kernel(const bf16* __restric
         sj[2 * i + 1] = kJScale * ((j1[0] + j1[1]) + j1[2]);
     }
     __syncthreads();
-    if (threadIdx.x < 256) {
+    if (threadIdx.x < TYO * 16) {
         const int r = threadIdx.x / 16, c = threadIdx.x % 16;  // tile pixel
         const int i = (r + 1) * JW + (c + 1), up = r * JW + (c + 1), left = (r + 1) * JW + c;
         const float div = (sj[2 * i] - sj[2 * up]) + (sj[2 * i + 1] - sj[2 * left + 1]);
@@ -1009,6 +1022,11 @@ struct amb_surrogate {
           *sumsq_part = nullptr;
     size_t part_len = 0, part2_len = 0;
     double* audit_part = nullptr;  // kAuditBlocks x (|a - b|^2, |b|^2)
+    float* audit_buf = nullptr;    // surrogate field of a pending overlapped audit
+    bool audit_pending = false;
+    int64_t audit_t0 = 0, audit_steps = 0;
+    amb_handle* audit_main = nullptr;
+    amb_handle* audit_judge = nullptr;
     float* p3 = nullptr;           // parameter triple of advance / audit (device)
     float last_p3[3] = {NAN, NAN, NAN};
     int* d_idx = nullptr;
@@ -1019,9 +1037,10 @@ struct amb_surrogate {
     float train_noise = 0.f;  // relative size of the training-input noise (0: off)
     uint64_t noise_seed = 0, noise_iter = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    hipGraphExec_t step_graph = nullptr;  // one surrogate step on a field, batch 1
-    void* graph_field = nullptr;
-    void* graph_out = nullptr;
+    // two-step graphs (field -> other -> field), one per buffer parity, batch 1
+    hipGraphExec_t step_graph[2] = {};
+    void* graph_field[2] = {};
+    void* graph_out[2] = {};
 };
 
 namespace {
@@ -1031,7 +1050,7 @@ std::vector<void*> all_buffers(amb_surrogate* s)
     std::vector<void*> v = {s->master, s->grad, s->adam_m, s->adam_v, s->x0, s->e1, s->e1p, s->p1, s->e2, s->e2p,
                             s->p2, s->bt, s->btp, s->u2, s->s2, s->d2, s->d2p, s->u1, s->s1, s->d1, s->d1p, s->dpre,
                             s->J, s->phi, s->target, s->params, s->norm, s->pred, s->g, s->dJ, s->dd1, s->ds1,
-                            s->dd2, s->ds2, s->dbt, s->dp2, s->dp1, s->part, s->part2, s->audit_part, s->p3,
+                            s->dd2, s->ds2, s->dbt, s->dp2, s->dp1, s->part, s->part2, s->audit_part, s->audit_buf, s->p3,
                             s->loss_part, s->sumsq_part, s->d_idx,
                             s->ds_in, s->ds_out, s->ds_par, s->ds_norm, s->wf0_8};
     for (int l = 0; l < 7; ++l) {
@@ -1045,7 +1064,8 @@ void release(amb_surrogate* s)
 {
     if (!s) return;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
-    if (s->step_graph) (void)hipGraphExecDestroy(s->step_graph);
+    for (hipGraphExec_t g : s->step_graph)
+        if (g) (void)hipGraphExecDestroy(g);
     for (void* p : all_buffers(s))
         if (p) (void)hipFree(p);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
@@ -1208,10 +1228,11 @@ int forward(amb_surrogate* s, int n, const float* phi, const float* params, floa
     return AMB_OK;
 }
 
-// Inference: phi_out = phi - div J in five launches: pooling, up-sampling adds and the
-// input layer are fused into the convolutions' LDS staging, the 1x1 layers into the
-// epilogue of the 3x3 layer before them, and c7 with the head and the flux update.
-// J_out (nullable) receives the current.
+// Inference: phi_out = phi - div J in five launches: c1 (input assembly); c2 (pooled
+// operand); c3 (pooled operand) with c4; c5 (up-sampled operand) with c6; c7
+// (up-sampled operand) with the head and the flux update.  Pooling, up-sampling adds and the input layer are applied
+// once per pixel while staging a tile in LDS; the 1x1 layers run in the epilogue of
+// the 3x3 layer before them.  J_out (nullable) receives the current.
 int forward_infer(amb_surrogate* s, int n, const float* phi, const float* params, float* phi_out,
                   float* J_out = nullptr)
 {
@@ -1231,8 +1252,8 @@ int forward_infer(amb_surrogate* s, int n, const float* phi, const float* params
     SUR_CHECK((conv_tile_launch<kSrcUpAdd, 64, 4, 2, 1, 4, 2>(ConvIn{s->u2, s->e2, nullptr, nullptr}, s->wf[4],
                                                                M + s->b_off[4], n, H2, W2, s->u1, st, s->wf[5],
                                                                M + s->b_off[5])));
-    hipLaunchKernelGGL(c7_head_flux_kernel, dim3(n * (H / 16) * (W / 16)), dim3(512), 0, st, s->u1, s->e1, s->wf[6],
-                       M + s->b_off[6], M + s->w_off[7], M + s->b_off[7], phi, H, W, phi_out, J_out);
+    hipLaunchKernelGGL(c7_head_flux_kernel<8>, dim3(n * (H / 8) * (W / 16)), dim3(512), 0, st, s->u1, s->e1,
+                       s->wf[6], M + s->b_off[6], M + s->w_off[7], M + s->b_off[7], phi, H, W, phi_out, J_out);
     SUR_CHECK(hipGetLastError());
     return AMB_OK;
 }
@@ -1451,6 +1472,7 @@ int amb_surrogate_create(int64_t ny, int64_t nx, int max_batch, uint64_t seed, a
     s->part2_len = s->part_len / kReduceGroup + 9 * 128 * 64;
     A(&s->part2, s->part2_len * 4);
     A(&s->audit_part, kAuditBlocks * 2 * sizeof(double));
+    A(&s->audit_buf, (size_t)ny * nx * sizeof(float));
     A(&s->p3, 4 * sizeof(float));
     A(&s->loss_part, (P1 / 256 + 1) * 4);
     A(&s->sumsq_part, 64 * 4);
@@ -1729,6 +1751,77 @@ int amb_surrogate_audit(amb_surrogate* s, amb_handle* main, amb_handle* judge, c
     return handle_commit(main, euler_steps, false);
 }
 
+int amb_surrogate_audit_begin(amb_surrogate* s, amb_handle* main, amb_handle* judge, const double* params3,
+                              int64_t euler_steps)
+{
+    clear_error();
+    if (!s || !main || !judge || !params3) return fail(AMB_E_ARG, "null argument");
+    if (main == judge) return fail(AMB_E_ARG, "main and judge must be different integrators");
+    if (euler_steps < 1) return fail(AMB_E_ARG, "euler_steps must be positive");
+    if (s->audit_pending) return fail(AMB_E_ARG, "an audit is already pending");
+    FieldView mv, jv;
+    int rc = handle_view(main, &mv);
+    if (rc == AMB_OK) rc = handle_view(judge, &jv);
+    if (rc == AMB_OK) rc = check_surrogate_view(s, mv);
+    if (rc == AMB_OK) rc = check_surrogate_view(s, jv);
+    if (rc != AMB_OK) return rc;
+    const size_t bytes = (size_t)s->H * s->W * sizeof(float);
+    SUR_CHECK(hipStreamSynchronize(mv.stream));
+    SUR_CHECK(hipStreamSynchronize(jv.stream));
+    // judge: the same start, euler_steps explicit-Euler steps on its own stream.  The
+    // copy completes before returning: the caller's next steps overwrite main's buffers.
+    SUR_CHECK(hipMemcpyAsync(jv.cur, mv.cur, bytes, hipMemcpyDeviceToDevice, jv.stream));
+    SUR_CHECK(hipStreamSynchronize(jv.stream));
+    if ((rc = handle_enqueue(judge, euler_steps)) != AMB_OK) return rc;
+    // surrogate: one step into main's other buffer, kept for the comparison
+    if ((rc = upload_params3(s, params3)) != AMB_OK) return rc;
+    if ((rc = forward_infer(s, 1, static_cast<float*>(mv.cur), s->p3, static_cast<float*>(mv.other))) != AMB_OK)
+        return rc;
+    SUR_CHECK(hipMemcpyAsync(s->audit_buf, mv.other, bytes, hipMemcpyDeviceToDevice, s->stream));
+    SUR_CHECK(hipStreamSynchronize(s->stream));
+    s->audit_t0 = handle_timestep(main);
+    s->audit_steps = euler_steps;
+    s->audit_main = main;
+    s->audit_judge = judge;
+    s->audit_pending = true;
+    return handle_commit(main, euler_steps, true);
+}
+
+int amb_surrogate_audit_end(amb_surrogate* s, double tol, double* rel_err, int* accepted)
+{
+    clear_error();
+    if (!s || !rel_err || !accepted) return fail(AMB_E_ARG, "null argument");
+    if (!s->audit_pending) return fail(AMB_E_ARG, "no audit is pending");
+    s->audit_pending = false;
+    amb_handle* main = s->audit_main;
+    amb_handle* judge = s->audit_judge;
+    FieldView mv, jv;
+    int rc = handle_view(main, &mv);
+    if (rc == AMB_OK) rc = handle_view(judge, &jv);
+    if (rc != AMB_OK) return rc;
+    SUR_CHECK(hipStreamSynchronize(jv.stream));
+    hipLaunchKernelGGL(audit_kernel, dim3(kAuditBlocks), dim3(256), 0, s->stream, s->audit_buf,
+                       static_cast<const float*>(jv.cur), s->H * s->W, s->audit_part);
+    SUR_CHECK(hipGetLastError());
+    double part[2 * kAuditBlocks];
+    SUR_CHECK(hipMemcpyAsync(part, s->audit_part, sizeof part, hipMemcpyDeviceToHost, s->stream));
+    SUR_CHECK(hipStreamSynchronize(s->stream));
+    double num = 0.0, den = 0.0;
+    for (int b = 0; b < kAuditBlocks; ++b) {
+        num += part[2 * b];
+        den += part[2 * b + 1];
+    }
+    const double err = std::sqrt(num) / std::sqrt(den);
+    *rel_err = err;
+    *accepted = err <= tol ? 1 : 0;  // false for NaN
+    if (*accepted) return AMB_OK;
+    // roll main back to the Euler field at the audited step, discarding later steps
+    SUR_CHECK(hipStreamSynchronize(mv.stream));
+    SUR_CHECK(hipMemcpyAsync(mv.cur, jv.cur, (size_t)s->H * s->W * sizeof(float), hipMemcpyDeviceToDevice,
+                             mv.stream));
+    return handle_set_timestep(main, s->audit_t0 + s->audit_steps);
+}
+
 // nsteps surrogate steps applied in place to the field of an FP32 integrator whose
 // grid matches; its step counter advances by nsteps * euler_steps_per_step.  The
 // per-step launches are captured once in a hipGraph.  device_ms (nullable): HIP-event
@@ -1747,10 +1840,15 @@ int amb_surrogate_advance(amb_surrogate* s, amb_handle* h, int64_t nsteps, const
     SUR_CHECK(hipStreamSynchronize(v.stream));
     if ((rc = upload_params3(s, params3)) != AMB_OK) return rc;
     const int P1 = s->H * s->W;
-    // Graph of two steps (field -> other -> field) so replays keep the buffer parity.
-    if (!s->step_graph || s->graph_field != v.cur || s->graph_out != v.other) {
-        if (s->step_graph) (void)hipGraphExecDestroy(s->step_graph);
-        s->step_graph = nullptr;
+    // Graph of two steps (field -> other -> field) so replays keep the buffer parity;
+    // one cached graph per parity of the integrator's buffers.
+    int slot = -1;
+    for (int i = 0; i < 2; ++i)
+        if (s->step_graph[i] && s->graph_field[i] == v.cur && s->graph_out[i] == v.other) slot = i;
+    if (slot < 0) {
+        slot = (s->step_graph[0] && s->graph_field[0] == v.other && s->graph_out[0] == v.cur) ? 1 : 0;
+        if (s->step_graph[slot]) (void)hipGraphExecDestroy(s->step_graph[slot]);
+        s->step_graph[slot] = nullptr;
         hipGraph_t g = nullptr;
         SUR_CHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
         int crc = forward_infer(s, 1, static_cast<float*>(v.cur), s->p3, static_cast<float*>(v.other));
@@ -1761,14 +1859,14 @@ int amb_surrogate_advance(amb_surrogate* s, amb_handle* h, int64_t nsteps, const
             return crc;
         }
         if (ee != hipSuccess) return fail_hip(ee, "hipStreamEndCapture");
-        const hipError_t ie = hipGraphInstantiate(&s->step_graph, g, nullptr, nullptr, 0);
+        const hipError_t ie = hipGraphInstantiate(&s->step_graph[slot], g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         if (ie != hipSuccess) return fail_hip(ie, "hipGraphInstantiate");
-        s->graph_field = v.cur;
-        s->graph_out = v.other;
+        s->graph_field[slot] = v.cur;
+        s->graph_out[slot] = v.other;
     }
     SUR_CHECK(hipEventRecord(s->ev0, s->stream));
-    for (int64_t k = 0; k + 1 < nsteps; k += 2) SUR_CHECK(hipGraphLaunch(s->step_graph, s->stream));
+    for (int64_t k = 0; k + 1 < nsteps; k += 2) SUR_CHECK(hipGraphLaunch(s->step_graph[slot], s->stream));
     if (nsteps & 1) {
         rc = forward_infer(s, 1, static_cast<float*>(v.cur), s->p3, static_cast<float*>(v.other));
         if (rc != AMB_OK) return rc;

@@ -349,7 +349,35 @@ class Surrogate {
         return py::make_tuple(err, accepted != 0);
     }
 
+    void audit_begin(py::object main, py::object judge, const std::vector<double>& params3, int64_t euler_steps)
+    {
+        if (params3.size() != 3) throw py::value_error("params3 = (lamb, zeta, phi0)");
+        amb_handle* hm = main.cast<Integrator&>().handle();
+        amb_handle* hj = judge.cast<Integrator&>().handle();
+        int rc;
+        {
+            py::gil_scoped_release release;
+            rc = amb_surrogate_audit_begin(s_, hm, hj, params3.data(), euler_steps);
+        }
+        check(rc);
+        pending_ = py::make_tuple(main, judge);  // keep both integrators alive until audit_end
+    }
+
+    py::tuple audit_end(double tol)
+    {
+        double err = 0;
+        int accepted = 0, rc;
+        {
+            py::gil_scoped_release release;
+            rc = amb_surrogate_audit_end(s_, tol, &err, &accepted);
+        }
+        pending_ = py::none();
+        check(rc);
+        return py::make_tuple(err, accepted != 0);
+    }
+
    private:
+    py::object pending_ = py::none();
     amb_surrogate* s_ = nullptr;
     int64_t ny_, nx_, np_ = 0;
     int max_batch_;
@@ -562,7 +590,10 @@ PYBIND11_MODULE(integrator, m)
         .def("advance", &Surrogate::advance, py::arg("integrator"), py::arg("nsteps"), py::arg("params3"),
              py::arg("euler_steps_per_step"))
         .def("audit", &Surrogate::audit, py::arg("main"), py::arg("judge"), py::arg("params3"),
-             py::arg("euler_steps"), py::arg("tol"));
+             py::arg("euler_steps"), py::arg("tol"))
+        .def("audit_begin", &Surrogate::audit_begin, py::arg("main"), py::arg("judge"), py::arg("params3"),
+             py::arg("euler_steps"))
+        .def("audit_end", &Surrogate::audit_end, py::arg("tol"));
 
     py::class_<Ensemble>(m, "Ensemble", py::dynamic_attr())
         .def(py::init<const std::vector<Field>&, const std::vector<Stencil>&, const std::vector<Model>&,

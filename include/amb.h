@@ -234,6 +234,18 @@ int amb_surrogate_advance(amb_surrogate* s, amb_handle* h, int64_t nsteps, const
 int amb_surrogate_audit(amb_surrogate* s, amb_handle* main, amb_handle* judge, const double* params3,
                         int64_t euler_steps, double tol, double* rel_err, int* accepted);
 
+/* The same audit overlapped with further work.  _begin enqueues the judge's
+ * euler_steps Euler steps on the judge's stream, takes the surrogate step on `main`
+ * (kept aside) and advances main's step counter; the caller may then advance `main`
+ * further (speculatively) while the judge runs.  _end waits for the judge and
+ * compares; if rel_err > tol (or not finite) `main` is rolled back to the Euler field
+ * at the audited step (counter = start + euler_steps), discarding the later steps.
+ * One audit may be pending per surrogate; neither integrator may be destroyed
+ * while it is. */
+int amb_surrogate_audit_begin(amb_surrogate* s, amb_handle* main, amb_handle* judge, const double* params3,
+                              int64_t euler_steps);
+int amb_surrogate_audit_end(amb_surrogate* s, double tol, double* rel_err, int* accepted);
+
 #ifdef __cplusplus
 }
 #endif

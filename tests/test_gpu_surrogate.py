@@ -145,3 +145,43 @@ def test_audit_accepts_rejects_and_falls_back(amb):
     assert not ok and not np.isfinite(err)
     np.testing.assert_array_equal(main.field, phi_e)
     assert abs(main.field.sum() - start.sum()) < 1e-3
+
+
+def test_overlapped_audit_rolls_back_speculative_steps(amb):
+    """audit_begin / audit_end: the judge runs while the surrogate carries on; a failed
+    audit restores exactly the Euler field and step counter of the audited step."""
+    S = amb.Surrogate(64, 64, max_batch=1, seed=15)
+    perturbed(S, 16)
+    rng = np.random.default_rng(17)
+    phi0 = 0.1 * rng.normal(size=(64, 64))
+    model, stencil = amb.Model(-0.25, 0, 0.25, 1, 0.5, -1, 0), amb.Stencil(5e-3, 1, 1)
+    main = amb.Integrator(phi0, stencil, model, seed=1, precision=32)
+    judge = amb.Integrator(phi0, stencil, model, seed=1, precision=32)
+    p3 = [0.5, -1.0, 0.0]
+    start = main.field
+    euler = amb.Integrator(start, stencil, model, seed=1, precision=32)
+    euler.run(100)
+    phi_e = euler.field
+    sync_err, _ = S.audit(amb.Integrator(start, stencil, model, seed=1, precision=32), judge, p3, 100, 1e9)
+
+    main.set_field(start)
+    S.audit_begin(main, judge, p3, 100)
+    S.advance(main, 3, p3, 100)
+    err, ok = S.audit_end(0.0)
+    assert not ok and err == pytest.approx(sync_err, rel=1e-12)
+    np.testing.assert_array_equal(main.field, phi_e)
+    assert main.timestep == 100
+
+    main.set_field(start)
+    t0 = main.timestep
+    S.audit_begin(main, judge, p3, 100)
+    S.advance(main, 3, p3, 100)
+    err, ok = S.audit_end(1e9)
+    assert ok
+    ref = start.copy()
+    for _ in range(4):
+        ref = S.predict(ref[None], np.array([p3]))[0][0]
+    np.testing.assert_array_equal(main.field, ref)
+    assert main.timestep == t0 + 400
+    with pytest.raises(ValueError):
+        S.audit_end(0.0)  # nothing pending
