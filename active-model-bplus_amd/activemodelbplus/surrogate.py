@@ -350,6 +350,9 @@ def _main(argv=None):
     t.add_argument("--seed", type=int, default=5)
     t.add_argument("--noise", type=float, default=0.3, help="relative training-input noise")
     t.add_argument("--val-realisations", type=int, default=8)
+    f = sub.add_parser("profile", help="inference steps and training iterations on synthetic data, for a profiler")
+    f.add_argument("--steps", type=int, default=200)
+    f.add_argument("--train-iterations", type=int, default=50)
     e = sub.add_parser("evaluate")
     e.add_argument("--weights", required=True)
     e.add_argument("--members", type=int, default=8)
@@ -363,7 +366,24 @@ def _main(argv=None):
     def log(msg):
         print(msg, flush=True)
 
-    if a.cmd == "train":
+    if a.cmd == "profile":
+        rng = np.random.default_rng(0)
+        sim = Integrator(0.1 * rng.normal(size=(256, 256)), Stencil(DT, 1, 1), Model(-0.25, 0, 0.25, 1, 0.5, -1, 0),
+                         seed=0, precision=32)
+        sur = Surrogate(256, 256, max_batch=1, seed=1)
+        if os.path.exists(default_weights_path()):
+            load_weights(sur, default_weights_path())
+        sur.advance(sim, 4, [0.5, -1.0, 0.0], DELTA_STEPS)
+        ms = sur.advance(sim, a.steps, [0.5, -1.0, 0.0], DELTA_STEPS)
+        log(f"inference: {ms / a.steps * 1e3:.1f} us per step (batch 1, 256^2)")
+        tr = Surrogate(256, 256, max_batch=16, seed=1)
+        phi = rng.normal(0, 0.3, (32, 256, 256)).astype(np.float32)
+        tr.set_dataset(phi, phi + 0.01, np.zeros((32, 3), np.float32))
+        tr.set_train_noise(0.3, 1)
+        t0 = time.perf_counter()
+        tr.train(a.train_iterations, 16, 1e-4, 1.0)
+        log(f"training: {(time.perf_counter() - t0) / a.train_iterations * 1e3:.2f} ms per iteration (batch 16, 256^2)")
+    elif a.cmd == "train":
         t0 = time.time()
         data = generate_pairs(a.realisations, 256, a.t_max, a.times, a.seed, log=log)
         log(f"dataset: {data[0].shape[0]} pairs in {time.time() - t0:.1f} s")
