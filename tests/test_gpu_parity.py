@@ -92,6 +92,48 @@ def test_config1_fp64_parity_1000_steps(amb, oracle):
     assert bad == 0 and abs(s - phi.sum()) < 1e-9
 
 
+def test_config1_fp64_parity_full_length(amb, oracle):
+    """Config 1 at its full length: passive spinodal, 256^2, FP64, dt=1e-3, 10^4 steps
+    vs the oracle (the CPU scheme on the host's cores)."""
+    phi = spinodal(256, 1)
+    st, md = (1e-3, 1, 1), (-0.25, 0, 0.25, 1, 0, 0, 0)
+    sim = make(amb, phi, st, md, seed=1)
+    sim.run(10_000)
+    ref = oracle.run(phi, st, md, 10_000, nthreads=min(16, oracle.max_threads()))
+    err = relmax(sim.field, ref)
+    print(f"config1 FP64 1e4 steps: rel max-norm error {err:.3e}")
+    assert err <= 1e-10
+    s, bad = sim.mass()
+    assert bad == 0 and abs(s - phi.sum()) < 1e-9
+
+
+def test_config2_fp32_against_fp64_full_length(amb):
+    """Config 2 at its full length (1024^2, lamb=-1, zeta=-4, T=0.1, dt=5e-3, 10^5 steps,
+    phi = -0.4 + 0.01 N(0,1)): the FP32 path against the FP64 path on the same Philox
+    stream.  The noisy trajectories decorrelate pointwise long before 10^5 steps, so the
+    comparison is statistical: mass, finiteness and the structure factor's first moment."""
+    from activemodelbplus import trajectory as T
+    rng = np.random.default_rng(2)
+    phi = -0.4 + 0.01 * rng.normal(size=(1024, 1024))
+    st, md = (5e-3, 1, 1), (-0.25, 0, 0.25, 1, -1, -4, 0.1)
+    out = {}
+    for prec in (64, 32):
+        sim = make(amb, phi, st, md, seed=2, precision=prec)
+        sim.run(100_000)
+        s, bad = sim.mass()
+        out[prec] = (sim, s, bad)
+        assert bad == 0
+    k64, k32 = T.mean_wavenumber(out[64][0]), T.mean_wavenumber(out[32][0])
+    f64, f32 = out[64][0].field, out[32][0].field
+    l2 = float(np.linalg.norm(f32 - f64) / np.linalg.norm(f64))
+    m0 = float(np.sum(phi))
+    print(f"config2 1e5 steps: <k> FP64 {k64:.4f} FP32 {k32:.4f} ({k32 / k64 - 1:+.3%}); "
+          f"L2(FP32 - FP64)/L2(FP64) {l2:.3e}; mass drift FP64 {out[64][1] - m0:.3e} FP32 {out[32][1] - m0:.3e}")
+    assert abs(out[64][1] - m0) < 1e-6
+    assert abs(out[32][1] - m0) / phi.size < 1e-5
+    assert abs(k32 / k64 - 1) < 0.05
+
+
 @pytest.mark.parametrize("shape,stencil,model", [
     ((64, 96), (1e-2, 1, 1), (-0.25, 0.5, 0.25, 1, 1, 1, 0)),
     ((48, 40), (5e-3, 1, 1.25), (-0.25, 0, 0.25, 1, -1, -4, 0)),
