@@ -162,13 +162,14 @@ class HybridIntegrator:
     """
 
     def __init__(self, sur, field, model, stencil=None, delta_steps=DELTA_STEPS, audit_every=20, tol=0.02,
-                 seed=0, max_audit_every=None):
+                 seed=0, max_audit_every=None, main=None, judge=None):
+        """``main`` / ``judge``: FP32 integrators to use (created from ``field`` if None)."""
         stencil = stencil or Stencil(DT, 1, 1)
         self.sur = sur
         self.model, self.stencil = model, stencil
         self.params3 = [model.lamb, model.zeta, float(np.mean(field))]
-        self.main = Integrator(field, stencil, model, seed=seed, precision=32)
-        self.judge = Integrator(field, stencil, model, seed=seed, precision=32)
+        self.main = main if main is not None else Integrator(field, stencil, model, seed=seed, precision=32)
+        self.judge = judge if judge is not None else Integrator(field, stencil, model, seed=seed, precision=32)
         self.delta, self.tol = delta_steps, tol
         self.audit_min = max(1, int(audit_every))
         self.audit_max = max(self.audit_min, int(max_audit_every or audit_every))
