@@ -1393,6 +1393,9 @@ int amb_surrogate_create(int64_t ny, int64_t nx, int max_batch, uint64_t seed, a
     int rc = check_dims(ny, nx);
     if (rc != AMB_OK) return rc;
     if (max_batch < 1 || max_batch > 256) return fail(AMB_E_ARG, "max_batch must be in [1, 256]");
+    // element counts of the 32-channel full-resolution tensors are 32-bit in the kernels
+    if ((int64_t)max_batch * ny * nx > ((int64_t)1 << 26))
+        return fail(AMB_E_ARG, "max_batch * ny * nx must not exceed 2^26");
     if ((rc = check_device()) != AMB_OK) return rc;
     amb_surrogate* s = new (std::nothrow) amb_surrogate();
     if (!s) return fail(AMB_E_ARG, "out of host memory");

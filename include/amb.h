@@ -189,6 +189,7 @@ int amb_structure_factor(amb_handle* h, const double* edges, int nedges, double*
  * floats); training uses Adam with optional gradient-norm clipping. */
 typedef struct amb_surrogate amb_surrogate;
 
+/* max_batch in [1, 256] with max_batch * ny * nx <= 2^26. */
 int amb_surrogate_create(int64_t ny, int64_t nx, int max_batch, uint64_t seed, amb_surrogate** out);
 int amb_surrogate_destroy(amb_surrogate* s);
 int amb_surrogate_param_count(const amb_surrogate* s, int64_t* n);

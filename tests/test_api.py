@@ -117,3 +117,14 @@ def test_integrator_is_subclassable():
     from activemodelbplus.integrator import integrator as core
     assert issubclass(Integrator, core.Integrator)
     assert hasattr(Integrator, "run_for_time") and hasattr(Integrator, "from_checkpoint")
+
+
+def test_surrogate_argument_errors():
+    """Shape and batch limits are checked before any device work (no GPU needed)."""
+    from activemodelbplus.integrator import Surrogate
+    with pytest.raises(ValueError):
+        Surrogate(100, 128)           # sides must be multiples of 64
+    with pytest.raises(ValueError):
+        Surrogate(256, 256, max_batch=0)
+    with pytest.raises(ValueError):
+        Surrogate(4096, 4096, max_batch=8)  # 2^27 pixels
