@@ -196,38 +196,34 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
     if (NOISE == kNoisePhilox || (NOISE == kNoiseRuntime && noisy)) {
         // One flat list of Philox calls: xi_x groups (rows [0,TH), tile-relative column
         // groups [-1, TW/4]) then xi_y groups (rows [-4,TH+4), groups [0, TW/4)).  Each
-        // thread runs PER independent calls, unrolled so their multiply chains overlap; one
-        // list rather than two keeps the idle lanes of the last round to one round.
+        // thread runs PER independent calls, unrolled so their multiply chains overlap.
         constexpr int GXN = TW / 4 + 2, GYN = TW / 4;
         constexpr int NXI = GXN * TH, NYI = GYN * L::YH, NITEMS = NXI + NYI;
+        constexpr int PERX = (NXI + NT - 1) / NT, PERY = (NYI + NT - 1) / NT;
         if (nx4 && interior) {
-            // no wrapping: xi_x rows [0, TH), column groups x0/4 - 1 .. x0/4 + TW/4, then
-            // xi_y rows [-4, TH+4), column groups x0/4 .. x0/4 + TW/4 - 1, as one list
-            constexpr int PER = (NITEMS + NT - 1) / NT;
+            // xi_x: rows [0, TH), column groups x0/4 - 1 .. x0/4 + TW/4
 #pragma unroll
-            for (int u = 0; u < PER; ++u) {
+            for (int u = 0; u < PERX; ++u) {
                 const int i0 = tid + u * NT;
-                const int i = i0 < NITEMS ? i0 : NITEMS - 1;
-                uint32_t group, grow;
-                int comp;
-                T* dst;
-                if (i < NXI) {
-                    const int r = i / GXN, gl = i - r * GXN;
-                    grow = (uint32_t)(y0 + r);
-                    group = (uint32_t)((x0 >> 2) + gl - 1);
-                    comp = 1;
-                    dst = s_xix + r * L::XW + 4 * gl;
-                } else {
-                    const int j = i - NXI;
-                    const int rr = j / GYN, gl = j - rr * GYN;
-                    grow = (uint32_t)(y0 + rr - 4);
-                    group = (uint32_t)((x0 >> 2) + gl);
-                    comp = 0;
-                    dst = s_xiy + rr * L::YW + 4 * gl;
-                }
+                const int i = i0 < NXI ? i0 : NXI - 1;
+                const int r = i / GXN, gl = i - r * GXN;
                 T z[4];
-                normals4<T>(group, grow, step, comp, key0, key1, z);
-                if (i0 < NITEMS) {
+                normals4<T>((uint32_t)((x0 >> 2) + gl - 1), (uint32_t)(y0 + r), step, 1, key0, key1, z);
+                if (i0 < NXI) {
+                    T* dst = s_xix + r * L::XW + 4 * gl;
+                    dst[0] = z[0]; dst[1] = z[1]; dst[2] = z[2]; dst[3] = z[3];
+                }
+            }
+            // xi_y: rows [-4, TH+4), column groups x0/4 .. x0/4 + TW/4 - 1
+#pragma unroll
+            for (int u = 0; u < PERY; ++u) {
+                const int i0 = tid + u * NT;
+                const int i = i0 < NYI ? i0 : NYI - 1;
+                const int rr = i / GYN, gl = i - rr * GYN;
+                T z[4];
+                normals4<T>((uint32_t)((x0 >> 2) + gl), (uint32_t)(y0 + rr - 4), step, 0, key0, key1, z);
+                if (i0 < NYI) {
+                    T* dst = s_xiy + rr * L::YW + 4 * gl;
                     dst[0] = z[0]; dst[1] = z[1]; dst[2] = z[2]; dst[3] = z[3];
                 }
             }
