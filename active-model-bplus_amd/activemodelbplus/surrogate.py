@@ -162,14 +162,17 @@ class HybridIntegrator:
     """
 
     def __init__(self, sur, field, model, stencil=None, delta_steps=DELTA_STEPS, audit_every=20, tol=0.02,
-                 seed=0, max_audit_every=None, main=None, judge=None):
-        """``main`` / ``judge``: FP32 integrators to use (created from ``field`` if None)."""
+                 seed=0, max_audit_every=None, main=None, judge=None, autotune=False):
+        """``main`` / ``judge``: FP32 integrators to use (created from ``field`` if None);
+        ``autotune``: pick their tile shape by timing (``Integrator.autotune``)."""
         stencil = stencil or Stencil(DT, 1, 1)
         self.sur = sur
         self.model, self.stencil = model, stencil
         self.params3 = [model.lamb, model.zeta, float(np.mean(field))]
         self.main = main if main is not None else Integrator(field, stencil, model, seed=seed, precision=32)
         self.judge = judge if judge is not None else Integrator(field, stencil, model, seed=seed, precision=32)
+        if autotune:
+            self.judge.set_tile(self.main.autotune(4))  # autotune keeps the fastest for main
         self.delta, self.tol = delta_steps, tol
         self.audit_min = max(1, int(audit_every))
         self.audit_max = max(self.audit_min, int(max_audit_every or audit_every))
@@ -282,7 +285,7 @@ def evaluate(sur, n_eval=8, side=256, t_max=1e4, seed=1234, audit_every=10, tol=
         for mode in ("hybrid", "surrogate_only"):
             if mode == "hybrid":
                 h = HybridIntegrator(sur, fields[i], models[i], st, audit_every=audit_every, tol=tol,
-                                     max_audit_every=max_audit_every)
+                                     max_audit_every=max_audit_every, autotune=True)
             else:
                 h = HybridIntegrator(sur, fields[i], models[i], st, audit_every=10**9, tol=1e9)
             m0 = float(h.field.sum())
@@ -304,9 +307,11 @@ def evaluate(sur, n_eval=8, side=256, t_max=1e4, seed=1234, audit_every=10, tol=
             log(f"  {mode} member {i}: {wall:6.2f} s, fallback {stats['fallback_steps']}/{stats['coarse_steps']}, "
                 f"<k> err at t={cps[-1]:g}: {out[str(cps[-1])]['k_mean'] / res['euler'][i][str(cps[-1])]['k_mean'] - 1:+.3f}")
 
-    # plain Euler wall time to t_max for one member (the baseline the hybrid replaces)
+    # plain Euler wall time to t_max for one member (the baseline the hybrid replaces),
+    # with its tile shape autotuned
     if euler_timing:
         sim = Integrator(fields[0], st, models[0], seed=0, precision=32)
+        sim.autotune(4)  # keeps the fastest compiled tile for this grid
         sim.run(200)
         n = int(round(t_max / DT))
         t0 = time.perf_counter()

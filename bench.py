@@ -242,13 +242,16 @@ def bench_surrogate(args):
         SG.load_weights(sur, weights)
     coarse = args.steps
     # warm-up: graphs, tiles and the audit path
-    w = SG.HybridIntegrator(sur, phi, model, st, audit_every=args.audit_every, max_audit_every=args.max_audit_every)
+    w = SG.HybridIntegrator(sur, phi, model, st, audit_every=args.audit_every, max_audit_every=args.max_audit_every,
+                            autotune=True)
     w.run_coarse(min(args.warmup, 2 * args.audit_every))
     sim = Integrator(phi, st, model, seed=0, precision=32)
+    sim.autotune(4)  # plain Euler at its fastest compiled tile for this grid (kept by autotune)
     sim.run(min(args.warmup, 2) * delta)
     sim.set_field(phi)
 
-    h = SG.HybridIntegrator(sur, phi, model, st, audit_every=args.audit_every, max_audit_every=args.max_audit_every)
+    h = SG.HybridIntegrator(sur, phi, model, st, audit_every=args.audit_every, max_audit_every=args.max_audit_every,
+                            autotune=True)
     t0 = time.perf_counter()
     h.run_coarse(coarse)
     f_h = h.field
