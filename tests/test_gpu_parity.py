@@ -183,6 +183,31 @@ def test_philox_noise_parity(amb, oracle, shape):
     assert sim.timestep == 217
 
 
+def _random_case(i):
+    """Case i of the randomized parity sweep (seeded): shape 1..80 per side, spacings,
+    model parameters in the ranges the reference's tests and examples use, T >= 0."""
+    rng = np.random.default_rng(1000 + i)
+    ny, nx = (int(v) for v in rng.integers(1, 81, size=2))
+    dx, dy = (float(v) for v in rng.choice([0.75, 1.0, 1.25], size=2))
+    dt = float(rng.choice([1e-3, 2e-3]))
+    model = (-0.25, float(rng.uniform(-0.5, 0.5)), 0.25, float(rng.uniform(0.5, 1.5)), float(rng.uniform(-2, 2)),
+             float(rng.uniform(-5, 1)), float(rng.choice([0.0, 0.05, 0.1])))
+    phi = float(rng.uniform(-0.5, 0.5)) + 0.05 * rng.normal(size=(ny, nx))
+    return phi, (dt, dx, dy), model, int(rng.integers(1, 2**40))
+
+
+@pytest.mark.parametrize("i", range(24))
+def test_randomized_fp64_parity(amb, oracle, i):
+    """Seeded random shapes (tiny, odd, seam-wrapping), spacings, parameters and
+    temperatures: 40 FP64 steps with the Philox stream on both sides vs the oracle."""
+    phi, st, md, seed = _random_case(i)
+    sim = make(amb, phi, st, md, seed=seed, timestep=i)
+    sim.run(40)
+    ref = oracle.run(phi, st, md, 40, seed=seed, step0=i)
+    err = relmax(sim.field, ref)
+    assert err <= 1e-10, (phi.shape, st, md, err)
+
+
 def test_fp32_against_fp64(amb, oracle):
     """FP32 state and arithmetic vs the FP64 oracle (noisy active config, 128^2)."""
     rng = np.random.default_rng(2)
