@@ -208,6 +208,27 @@ def test_randomized_fp64_parity(amb, oracle, i):
     assert err <= 1e-10, (phi.shape, st, md, err)
 
 
+@pytest.mark.parametrize("precision", [32, 64])
+def test_randomized_ensemble_members_equal_single_runs(amb, precision):
+    """12 members of random sizes (8..100 per side, some not multiples of 4), parameters,
+    temperatures and seeds advanced together equal plain integrators bit for bit."""
+    rng = np.random.default_rng(77 + precision)
+    fields, models, seeds = [], [], []
+    for _ in range(12):
+        ny, nx = (int(v) for v in rng.integers(8, 101, size=2))
+        fields.append(float(rng.uniform(-0.5, 0.5)) + 0.05 * rng.normal(size=(ny, nx)))
+        models.append(amb.Model(-0.25, 0, 0.25, 1, float(rng.uniform(-2, 2)), float(rng.uniform(-5, 0)),
+                                float(rng.choice([0.0, 0.05, 0.1]))))
+        seeds.append(int(rng.integers(1, 2**40)))
+    st = amb.Stencil(2e-3, 1, 1)
+    ens = amb.Ensemble(fields, st, models, seeds, precision=precision)
+    ens.run(30)
+    for i in range(12):
+        sim = amb.Integrator(fields[i], st, models[i], seed=seeds[i], precision=precision)
+        sim.run(30)
+        np.testing.assert_array_equal(ens.field(i), sim.field, err_msg=f"member {i} {fields[i].shape}")
+
+
 def test_fp32_against_fp64(amb, oracle):
     """FP32 state and arithmetic vs the FP64 oracle (noisy active config, 128^2)."""
     rng = np.random.default_rng(2)
