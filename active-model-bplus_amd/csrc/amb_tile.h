@@ -38,7 +38,8 @@ struct Lds {
     static constexpr size_t bytes = sizeof(T) * (size_t)total;
     static_assert(NT % TW == 0, "tile width must divide the workgroup");
     static_assert((TW * TH) % NT == 0, "tile must be a multiple of the workgroup");
-    static_assert(NT >= MW, "workgroup must cover one stage-A row of the halo region");
+    static_assert(NT >= TW + 4, "workgroup must cover one stage-A row of the box");
+    static_assert(NT >= 2 * (TW + TH), "workgroup must cover the four stage-A arms");
     static_assert(TW % 4 == 0, "noise column groups need TW % 4 == 0");
 };
 
@@ -83,6 +84,24 @@ __device__ __forceinline__ T normal1(int row, int col, uint64_t step, int comp, 
     normals4<T>((uint32_t)(col >> 2), (uint32_t)row, step, comp, k0, k1, z);
     const int l = col & 3;
     return l == 0 ? z[0] : l == 1 ? z[1] : l == 2 ? z[2] : z[3];
+}
+
+// ---- local terms -------------------------------------------------------------
+
+// L3 Laplacian from the centre m0, the sum of its x-neighbours xs, of its
+// y-neighbours ys and of its four corners cs; and mu_p = m0 (a + m0 (b + m0 c)) -
+// kappa lap.  Explicit fma fixes the rounding, so every stage-A path that evaluates
+// them (box strips, arms) gives the same bits for the same site.
+template <typename T>
+__device__ __forceinline__ T laplacian3(const StepCoef<T>& K, T m0, T xs, T ys, T cs)
+{
+    return fma(K.lck, cs, fma(K.ley, ys, fma(K.lex, xs, K.lcc * m0)));
+}
+
+template <typename T>
+__device__ __forceinline__ T mu_passive(const StepCoef<T>& K, T m0, T lap)
+{
+    return fma(-K.kappa, lap, m0 * fma(m0, fma(m0, K.c, K.b), K.a));
 }
 
 // ---- fused step --------------------------------------------------------------
@@ -145,46 +164,82 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
 #pragma unroll
     for (int k = 0; k < R; ++k) phi_c[k] = s_phi[(rg * R + k + 9) * L::PW + cx + 9];
 
-    // ---- stage A: local terms on the (TH+16) x (TW+16) halo region, marched down
-    // column strips so each phi row is read once and its horizontal difference and
-    // sum are reused by the three output rows that need them.
+    // ---- stage A: local terms.  Stage B reads mu_a on the box rows/cols [-2, T+2), the
+    // circulating current on [-1, T+1), and mu_p on a cross: that box plus four arms six
+    // sites deep (rows [-8,-2) and [TH+2,TH+8) over the tile's columns, columns [-8,-2)
+    // and [TW+2,TW+8) over its rows); the corners of the halo region are never read.
+    // The box is marched down column strips with every term, so each phi row is read
+    // once and its horizontal difference and sum serve the three rows that need them;
+    // the arms get mu_p only, marched along their length.
     {
-        constexpr int NSEG = NT / L::MW;
-        constexpr int SEG = (L::MH + NSEG - 1) / NSEG;
-        if (tid < NSEG * L::MW) {
-            const int seg = tid / L::MW;
-            const int cc = tid - seg * L::MW;  // region column, c = cc - 8
-            const int rr0 = seg * SEG;         // region row, r = rr - 8
-            const T* P = s_phi + rr0 * L::PW + cc;  // phi at (r0 - 1, c - 1)
+        constexpr int BW = TW + 4, BH = TH + 4;  // box: br = r + 2, bc = c + 2
+        constexpr int NSEG = NT / BW;
+        constexpr int SEG = (BH + NSEG - 1) / NSEG;
+        const int seg = tid / BW;
+        if (seg * SEG < BH) {
+            const int bc = tid - seg * BW;
+            const int br0 = seg * SEG;
+            const T* P = s_phi + (br0 + 6) * L::PW + bc + 6;  // phi at (r0 - 1, c - 1)
             T tm = P[0], t0 = P[1], tp = P[2];
             T mm = P[L::PW], m0 = P[L::PW + 1], mp = P[L::PW + 2];
             T th1 = tp - tm, th2 = tp + tm;
             T mh1 = mp - mm, mh2 = mp + mm;
-            const int c = cc - 8;
-            const bool cbox2 = (unsigned)(c + 2) < (unsigned)(TW + 4);
-            const bool cbox1 = (unsigned)(c + 1) < (unsigned)(TW + 2);
+            const bool qcol = (unsigned)(bc - 1) < (unsigned)(TW + 2);
 #pragma unroll
             for (int k = 0; k < SEG; ++k) {
-                const int rr = rr0 + k;
-                if (rr < L::MH) {
+                const int br = br0 + k;
+                if (br < BH) {
                     const T* Q = P + (k + 2) * L::PW;
                     const T bm = Q[0], b0 = Q[1], bp = Q[2];
                     const T bh1 = bp - bm, bh2 = bp + bm;
                     const T gx = K.gxe * (th1 + bh1) + K.gxc * mh1;
                     const T gy = K.gye * (bh2 - th2) + K.gyc * (b0 - t0);
-                    const T lap = K.lcc * m0 + K.lex * mh2 + K.ley * (t0 + b0) + K.lck * (th2 + bh2);
-                    s_mup[rr * L::MW + cc] = m0 * (K.a + m0 * (K.b + m0 * K.c)) - K.kappa * lap;
-                    const int r = rr - 8;
-                    if (cbox2 && (unsigned)(r + 2) < (unsigned)(TH + 4))
-                        s_mua[(r + 2) * L::AW + (c + 2)] = K.lam * (gy * gy + gx * gx);
-                    if (cbox1 && (unsigned)(r + 1) < (unsigned)(TH + 2)) {
+                    const T lap = laplacian3(K, m0, mh2, t0 + b0, th2 + bh2);
+                    s_mup[(br + 6) * L::MW + bc + 6] = mu_passive(K, m0, lap);
+                    s_mua[br * L::AW + bc] = K.lam * (gy * gy + gx * gx);
+                    if (qcol && (unsigned)(br - 1) < (unsigned)(TH + 2)) {
                         const T zl = K.zeta * lap;
-                        s_qx[(r + 1) * L::QW + (c + 1)] = zl * gx;
-                        s_qy[(r + 1) * L::QW + (c + 1)] = zl * gy;
+                        s_qx[(br - 1) * L::QW + bc - 1] = zl * gx;
+                        s_qy[(br - 1) * L::QW + bc - 1] = zl * gy;
                     }
                     tm = mm; t0 = m0; tp = mp; th1 = mh1; th2 = mh2;
                     mm = bm; m0 = b0; mp = bp; mh1 = bh1; mh2 = bh2;
                 }
+            }
+        }
+        constexpr int ARM = 6;
+        if (tid < 2 * TW) {
+            // top or bottom arm: one column, marched down
+            const int c = tid % TW;
+            const int r0 = tid < TW ? -8 : TH + 2;
+            const T* P = s_phi + (r0 + 8) * L::PW + c + 8;  // phi at (r0 - 1, c - 1)
+            T t0 = P[1], th2 = P[2] + P[0];
+            T m0 = P[L::PW + 1], mh2 = P[L::PW + 2] + P[L::PW];
+            T* M = s_mup + (r0 + 8) * L::MW + c + 8;
+#pragma unroll
+            for (int k = 0; k < ARM; ++k) {
+                const T* Q = P + (k + 2) * L::PW;
+                const T b0 = Q[1], bh2 = Q[2] + Q[0];
+                M[k * L::MW] = mu_passive(K, m0, laplacian3(K, m0, mh2, t0 + b0, th2 + bh2));
+                t0 = m0; th2 = mh2;
+                m0 = b0; mh2 = bh2;
+            }
+        } else if (tid < 2 * (TW + TH)) {
+            // left or right arm: one row, marched right
+            const int t = tid - 2 * TW;
+            const int r = t % TH;
+            const int c0 = t < TH ? -8 : TW + 2;
+            const T* P = s_phi + (r + 8) * L::PW + c0 + 8;  // phi at (r - 1, c0 - 1)
+            T lt = P[0], lm = P[L::PW], lb = P[2 * L::PW];
+            T ct = P[1], cm = P[L::PW + 1], cb = P[2 * L::PW + 1];
+            T* M = s_mup + (r + 8) * L::MW + c0 + 8;
+#pragma unroll
+            for (int k = 0; k < ARM; ++k) {
+                const T* Q = P + k + 2;
+                const T rt = Q[0], rm = Q[L::PW], rb = Q[2 * L::PW];
+                M[k] = mu_passive(K, cm, laplacian3(K, cm, rm + lm, ct + cb, (rt + lt) + (rb + lb)));
+                lt = ct; lm = cm; lb = cb;
+                ct = rt; cm = rm; cb = rb;
             }
         }
     }
