@@ -39,7 +39,7 @@ struct Lds {
     static_assert(NT % TW == 0, "tile width must divide the workgroup");
     static_assert((TW * TH) % NT == 0, "tile must be a multiple of the workgroup");
     static_assert(NT >= TW + 4, "workgroup must cover one stage-A row of the box");
-    static_assert(NT >= 2 * (TW + TH), "workgroup must cover the four stage-A arms");
+    static_assert(TH % 8 == 0 && NT >= 2 * TW + 12 * (TH / 8), "workgroup must cover the four stage-A arms");
     static_assert(TW % 4 == 0, "noise column groups need TW % 4 == 0");
 };
 
@@ -102,6 +102,26 @@ template <typename T>
 __device__ __forceinline__ T mu_passive(const StepCoef<T>& K, T m0, T lap)
 {
     return fma(-K.kappa, lap, m0 * fma(m0, fma(m0, K.c, K.b), K.a));
+}
+
+// mu_p at rows r0 .. r0+N-1 of tile column c (tile-relative; LDS phi holds rows and
+// columns from -9, mu_p from -8), marched down: each phi row is read once and its
+// horizontal sum serves the three rows that need it.
+template <typename T, int N, int PW, int MW>
+__device__ __forceinline__ void mu_column(const T* s_phi, T* s_mup, const StepCoef<T>& K, const int r0, const int c)
+{
+    const T* P = s_phi + (r0 + 8) * PW + c + 8;  // phi at (r0 - 1, c - 1)
+    T t0 = P[1], th2 = P[2] + P[0];
+    T m0 = P[PW + 1], mh2 = P[PW + 2] + P[PW];
+    T* M = s_mup + (r0 + 8) * MW + c + 8;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const T* Q = P + (k + 2) * PW;
+        const T b0 = Q[1], bh2 = Q[2] + Q[0];
+        M[k * MW] = mu_passive(K, m0, laplacian3(K, m0, mh2, t0 + b0, th2 + bh2));
+        t0 = m0; th2 = mh2;
+        m0 = b0; mh2 = bh2;
+    }
 }
 
 // ---- fused step --------------------------------------------------------------
@@ -170,7 +190,7 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
     // and [TW+2,TW+8) over its rows); the corners of the halo region are never read.
     // The box is marched down column strips with every term, so each phi row is read
     // once and its horizontal difference and sum serve the three rows that need them;
-    // the arms get mu_p only, marched along their length.
+    // the arms get mu_p only.
     {
         constexpr int BW = TW + 4, BH = TH + 4;  // box: br = r + 2, bc = c + 2
         constexpr int NSEG = NT / BW;
@@ -207,40 +227,16 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
                 }
             }
         }
-        constexpr int ARM = 6;
+        // Arms, marched down columns so that neighbouring lanes touch neighbouring
+        // words: top and bottom arms one 6-row column per thread; the left and right arms
+        // (6 columns each) in 8-row segments, 12 threads per segment.
         if (tid < 2 * TW) {
-            // top or bottom arm: one column, marched down
-            const int c = tid % TW;
-            const int r0 = tid < TW ? -8 : TH + 2;
-            const T* P = s_phi + (r0 + 8) * L::PW + c + 8;  // phi at (r0 - 1, c - 1)
-            T t0 = P[1], th2 = P[2] + P[0];
-            T m0 = P[L::PW + 1], mh2 = P[L::PW + 2] + P[L::PW];
-            T* M = s_mup + (r0 + 8) * L::MW + c + 8;
-#pragma unroll
-            for (int k = 0; k < ARM; ++k) {
-                const T* Q = P + (k + 2) * L::PW;
-                const T b0 = Q[1], bh2 = Q[2] + Q[0];
-                M[k * L::MW] = mu_passive(K, m0, laplacian3(K, m0, mh2, t0 + b0, th2 + bh2));
-                t0 = m0; th2 = mh2;
-                m0 = b0; mh2 = bh2;
-            }
-        } else if (tid < 2 * (TW + TH)) {
-            // left or right arm: one row, marched right
+            mu_column<T, 6, L::PW, L::MW>(s_phi, s_mup, K, tid < TW ? -8 : TH + 2, tid % TW);
+        } else if (tid < 2 * TW + 12 * (TH / 8)) {
             const int t = tid - 2 * TW;
-            const int r = t % TH;
-            const int c0 = t < TH ? -8 : TW + 2;
-            const T* P = s_phi + (r + 8) * L::PW + c0 + 8;  // phi at (r - 1, c0 - 1)
-            T lt = P[0], lm = P[L::PW], lb = P[2 * L::PW];
-            T ct = P[1], cm = P[L::PW + 1], cb = P[2 * L::PW + 1];
-            T* M = s_mup + (r + 8) * L::MW + c0 + 8;
-#pragma unroll
-            for (int k = 0; k < ARM; ++k) {
-                const T* Q = P + k + 2;
-                const T rt = Q[0], rm = Q[L::PW], rb = Q[2 * L::PW];
-                M[k] = mu_passive(K, cm, laplacian3(K, cm, rm + lm, ct + cb, (rt + lt) + (rb + lb)));
-                lt = ct; lm = cm; lb = cb;
-                ct = rt; cm = rm; cb = rb;
-            }
+            const int q = t % 12;  // 0..5 left arm, 6..11 right arm
+            const int c = q < 6 ? q - 8 : TW + 2 + (q - 6);
+            mu_column<T, 8, L::PW, L::MW>(s_phi, s_mup, K, (t / 12) * 8, c);
         }
     }
     __syncthreads();

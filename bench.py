@@ -14,9 +14,11 @@ launch duration from HIP events on the integrator's own stream.  ``cpu_baseline`
 times the CPU FP64 oracle (oracle/, the checker) on this host on a bounded sample.
 
 Other workloads (--workload): c1 (256^2 FP64 spinodal), c2 (1024^2 FP32 noisy
-active), c3 (default), c3f64 (4096^2 FP64), c4 (batched ensemble of 512
+active), c3 (default), c3f64 (4096^2 FP64), c3x8192 (the C3 model on an 8192^2 FP32
+field: phi^n and phi^(n+1) together are 512 MiB, past the 256 MB MALL), c4 (batched ensemble of 512
 realisations, sides 64..512, vs sequential plain runs on the <=256^2 subset), c5
-(learned surrogate with Euler audits vs plain Euler, 256^2; --steps = coarse steps).
+(learned surrogate with Euler audits vs plain Euler, 256^2; --steps = coarse steps of
+100 dt each, default 20000 = t 10^4, the horizon the speedup is quoted at).
 """
 import argparse
 import json
@@ -31,7 +33,7 @@ sys.path.insert(0, os.path.join(REPO, "active-model-bplus_amd"))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_PROFILE = {"c3": "step_c3_fp32_4096"}
+TRAFFIC_PROFILE = {"c3": "step_c3_fp32_4096", "c3x8192": "step_c3_fp32_8192"}
 METRIC = "site-updates/s, % HBM peak; batched vs plain; surrogate speedup at ≤2% S(k) err"
 
 WORKLOADS = {
@@ -43,6 +45,8 @@ WORKLOADS = {
                init=("normal", 0.0, 0.05), label="C3 AMB+ noisy 4096x4096 FP32 (roofline)"),
     "c3f64": dict(n=4096, precision=64, seed=3, stencil=(5e-3, 1, 1), model=(-0.25, 0, 0.25, 1, 0.5, -1, 0.05),
                   init=("normal", 0.0, 0.05), label="C3 AMB+ noisy 4096x4096 FP64"),
+    "c3x8192": dict(n=8192, precision=32, seed=3, stencil=(5e-3, 1, 1), model=(-0.25, 0, 0.25, 1, 0.5, -1, 0.05),
+                    init=("normal", 0.0, 0.05), label="C3 AMB+ noisy 8192x8192 FP32 (beyond the MALL)"),
 }
 
 
@@ -56,8 +60,19 @@ def initial_field(w):
     return mean + std * rng.normal(size=(n, n))
 
 
-def cpu_baseline(w, phi, budget_s=15.0):
-    """Times the FP64 CPU oracle (OpenMP over rows) on this host."""
+def host_cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(w, phi, budget_s=15.0, budget_1core_s=8.0):
+    """Times the FP64 CPU oracle (OpenMP over rows) on this host: with up to 16 threads
+    (``value``, ``cores``) and on one thread (``value_1core``)."""
     from oracle import oracle as o
 
     o.lib()
@@ -68,16 +83,23 @@ def cpu_baseline(w, phi, budget_s=15.0):
     while side > 256 and (side * side) > 4e6:
         side //= 2
     sub = np.ascontiguousarray(phi[:side, :side])
-    t0 = time.perf_counter()
-    o.run(sub, w["stencil"], w["model"], 1, seed=w["seed"], nthreads=threads)
-    one = time.perf_counter() - t0
-    steps = max(1, int(budget_s / max(one, 1e-6)))
-    t0 = time.perf_counter()
-    o.run(sub, w["stencil"], w["model"], steps, seed=w["seed"], nthreads=threads)
-    dt = time.perf_counter() - t0
-    return {"value": side * side * steps / dt, "unit": "site-updates/s", "cores": threads, "kind": "port",
+
+    def rate(nthreads, budget):
+        t0 = time.perf_counter()
+        o.run(sub, w["stencil"], w["model"], 1, seed=w["seed"], nthreads=nthreads)
+        one = time.perf_counter() - t0
+        steps = max(1, int(budget / max(one, 1e-6)))
+        t0 = time.perf_counter()
+        o.run(sub, w["stencil"], w["model"], steps, seed=w["seed"], nthreads=nthreads)
+        dt = time.perf_counter() - t0
+        return side * side * steps / dt, steps, dt
+
+    value, steps, dt = rate(threads, budget_s)
+    value1, steps1, dt1 = rate(1, budget_1core_s)
+    return {"value": value, "unit": "site-updates/s", "cores": threads, "kind": "port",
             "sample": f"oracle FP64, {side}x{side} periodic sub-grid of the same workload, {steps} steps, "
-                      f"{threads} OpenMP threads, {dt:.1f} s"}
+                      f"{threads} OpenMP threads, {dt:.1f} s; 1 thread: {steps1} steps, {dt1:.1f} s",
+            "value_1core": value1, "host_cpu": f"{host_cpu_model()}, {os.cpu_count()} logical CPUs visible"}
 
 
 def read_traffic(name, tile):
@@ -197,11 +219,13 @@ def bench_batched(args):
 
 
 def bench_copy(args):
-    """Plain device copy of a 4096 x 4096 FP32 array (read 64 MiB + write 64 MiB per
-    launch): the calibration kernel for the rocprofv3 FETCH_SIZE / WRITE_SIZE counters."""
+    """Plain device copy of a side x side FP32 array (--copy-side, default 4096: read
+    64 MiB + write 64 MiB per launch): the calibration kernel for the rocprofv3
+    FETCH_SIZE / WRITE_SIZE counters."""
     import torch
 
-    src = torch.randn(4096, 4096, device="cuda")
+    side = args.copy_side
+    src = torch.randn(side, side, device="cuda")
     dst = torch.empty_like(src)
     for _ in range(max(1, args.warmup)):
         dst.copy_(src)
@@ -215,7 +239,7 @@ def bench_copy(args):
     return {"metric": "copy GB/s (counter calibration)", "value": nbytes * args.steps / wall / 1e9, "unit": "GB/s",
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-            "data": "synthetic", "config": {"workload": "plain copy 4096x4096 FP32 (torch)",
+            "data": "synthetic", "config": {"workload": f"plain copy {side}x{side} FP32 (torch)",
                                               "bytes_per_launch": nbytes}}
 
 
@@ -288,16 +312,19 @@ def bench_surrogate(args):
 def main():
     p = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=2000)
+    p.add_argument("--steps", type=int, default=None, help="timed steps (default 2000; c5: 20000 coarse steps)")
     p.add_argument("--warmup", type=int, default=64)
     p.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c4", "c5", "copy"])
     p.add_argument("--audit-every", type=int, default=10, help="c5: initial coarse steps per Euler audit")
     p.add_argument("--max-audit-every", type=int, default=200, help="c5: longest audit interval")
     p.add_argument("--realisations", type=int, default=512)
+    p.add_argument("--copy-side", type=int, default=4096, help="copy: side of the square FP32 array")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     p.add_argument("--autotune", action="store_true", help="pick the tile shape by timing")
     p.add_argument("--tile", type=int, default=0, help="0 = default tile, k > 0 = tile index k-1")
     args = p.parse_args()
+    if args.steps is None:
+        args.steps = 20000 if args.workload == "c5" else 2000
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU slab "
               "decomposition is a design, not a measured path.", file=sys.stderr)
