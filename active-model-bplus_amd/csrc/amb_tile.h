@@ -252,28 +252,30 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
         constexpr int NXI = GXN * TH, NYI = GYN * L::YH, NITEMS = NXI + NYI;
         constexpr int PERX = (NXI + NT - 1) / NT, PERY = (NYI + NT - 1) / NT;
         if (nx4 && interior) {
+            // Items past the end of a list are skipped, not recomputed: a wave whose lanes
+            // are all past it branches over the call.  The xi_y items are dealt from thread
+            // 64 on, so the two lists' last, partial rounds fall on different waves.
             // xi_x: rows [0, TH), column groups x0/4 - 1 .. x0/4 + TW/4
 #pragma unroll
             for (int u = 0; u < PERX; ++u) {
-                const int i0 = tid + u * NT;
-                const int i = i0 < NXI ? i0 : NXI - 1;
-                const int r = i / GXN, gl = i - r * GXN;
-                T z[4];
-                normals4<T>((uint32_t)((x0 >> 2) + gl - 1), (uint32_t)(y0 + r), step, 1, key0, key1, z);
-                if (i0 < NXI) {
+                const int i = tid + u * NT;
+                if ((u + 1) * NT <= NXI || i < NXI) {
+                    const int r = i / GXN, gl = i - r * GXN;
+                    T z[4];
+                    normals4<T>((uint32_t)((x0 >> 2) + gl - 1), (uint32_t)(y0 + r), step, 1, key0, key1, z);
                     T* dst = s_xix + r * L::XW + 4 * gl;
                     dst[0] = z[0]; dst[1] = z[1]; dst[2] = z[2]; dst[3] = z[3];
                 }
             }
             // xi_y: rows [-4, TH+4), column groups x0/4 .. x0/4 + TW/4 - 1
+            const int ty = tid >= 64 ? tid - 64 : tid + NT - 64;
 #pragma unroll
             for (int u = 0; u < PERY; ++u) {
-                const int i0 = tid + u * NT;
-                const int i = i0 < NYI ? i0 : NYI - 1;
-                const int rr = i / GYN, gl = i - rr * GYN;
-                T z[4];
-                normals4<T>((uint32_t)((x0 >> 2) + gl), (uint32_t)(y0 + rr - 4), step, 0, key0, key1, z);
-                if (i0 < NYI) {
+                const int i = ty + u * NT;
+                if ((u + 1) * NT <= NYI || i < NYI) {
+                    const int rr = i / GYN, gl = i - rr * GYN;
+                    T z[4];
+                    normals4<T>((uint32_t)((x0 >> 2) + gl), (uint32_t)(y0 + rr - 4), step, 0, key0, key1, z);
                     T* dst = s_xiy + rr * L::YW + 4 * gl;
                     dst[0] = z[0]; dst[1] = z[1]; dst[2] = z[2]; dst[3] = z[3];
                 }
