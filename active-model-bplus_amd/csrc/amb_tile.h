@@ -190,11 +190,23 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
     // and [TW+2,TW+8) over its rows); the corners of the halo region are never read.
     // The box is marched down column strips with every term, so each phi row is read
     // once and its horizontal difference and sum serve the three rows that need them;
-    // the arms get mu_p only.
+    // the arms get mu_p only, marched down columns so that neighbouring lanes touch
+    // neighbouring words: the top and bottom arms one 6-row column per thread, the left
+    // and right arms (6 columns each) in 8-row segments, 12 threads per segment.
+    // Two layouts, chosen per tile shape by the busiest wave's work (a box row costs
+    // about two arm rows): every thread on the box with the arms added to the first
+    // waves, or the last wave on the arms and longer box strips on the others.
     {
         constexpr int BW = TW + 4, BH = TH + 4;  // box: br = r + 2, bc = c + 2
-        constexpr int NSEG = NT / BW;
-        constexpr int SEG = (BH + NSEG - 1) / NSEG;
+        constexpr int NARM_TB = 2 * TW, NARM_LR = 12 * (TH / 8);
+        constexpr int SEG_ALL = (BH + NT / BW - 1) / (NT / BW);
+        constexpr int NSEG_RES = (NT - 64) / BW;
+        constexpr int SEG_RES = NSEG_RES > 0 ? (BH + NSEG_RES - 1) / NSEG_RES : BH;
+        constexpr int ARM_ROWS_RES = 6 * ((NARM_TB + 63) / 64) + 8 * ((NARM_LR + 63) / 64);
+        constexpr int COST_ALL = 2 * SEG_ALL + 8;
+        constexpr int COST_RES = 2 * SEG_RES > ARM_ROWS_RES ? 2 * SEG_RES : ARM_ROWS_RES;
+        constexpr bool ARM_WAVE = NSEG_RES > 0 && COST_RES < COST_ALL;
+        constexpr int SEG = ARM_WAVE ? SEG_RES : SEG_ALL;
         const int seg = tid / BW;
         if (seg * SEG < BH) {
             const int bc = tid - seg * BW;
@@ -227,16 +239,24 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
                 }
             }
         }
-        // Arms, marched down columns so that neighbouring lanes touch neighbouring
-        // words: top and bottom arms one 6-row column per thread; the left and right arms
-        // (6 columns each) in 8-row segments, 12 threads per segment.
-        if (tid < 2 * TW) {
+        if constexpr (ARM_WAVE) {
+            if (tid >= NT - 64) {
+                const int lane = tid - (NT - 64);
+#pragma unroll
+                for (int a = lane; a < NARM_TB; a += 64)
+                    mu_column<T, 6, L::PW, L::MW>(s_phi, s_mup, K, a < TW ? -8 : TH + 2, a % TW);
+#pragma unroll
+                for (int a = lane; a < NARM_LR; a += 64) {
+                    const int q = a % 12;  // 0..5 left arm, 6..11 right arm
+                    mu_column<T, 8, L::PW, L::MW>(s_phi, s_mup, K, (a / 12) * 8, q < 6 ? q - 8 : TW + 2 + (q - 6));
+                }
+            }
+        } else if (tid < NARM_TB) {
             mu_column<T, 6, L::PW, L::MW>(s_phi, s_mup, K, tid < TW ? -8 : TH + 2, tid % TW);
-        } else if (tid < 2 * TW + 12 * (TH / 8)) {
-            const int t = tid - 2 * TW;
-            const int q = t % 12;  // 0..5 left arm, 6..11 right arm
-            const int c = q < 6 ? q - 8 : TW + 2 + (q - 6);
-            mu_column<T, 8, L::PW, L::MW>(s_phi, s_mup, K, (t / 12) * 8, c);
+        } else if (tid < NARM_TB + NARM_LR) {
+            const int t = tid - NARM_TB;
+            const int q = t % 12;
+            mu_column<T, 8, L::PW, L::MW>(s_phi, s_mup, K, (t / 12) * 8, q < 6 ? q - 8 : TW + 2 + (q - 6));
         }
     }
     __syncthreads();
