@@ -165,7 +165,8 @@ def sim_tile_shape(precision, index):
 
 def bench_batched(args):
     """C4: 512 realisations, side in {64,128,256,512}, lambda~U[-2,2], zeta~U[-5,0],
-    phi0~U[-0.6,0.6], T in {0,0.05,0.1}; batched vs sequential plain on <=256^2."""
+    phi0~U[-0.6,0.6], T in {0,0.05,0.1}; batched vs sequential plain runs, on the <=256^2
+    subset and on all realisations (plain times extrapolated from --steps/10 steps each)."""
     from activemodelbplus.integrator import Ensemble, Integrator, Model, Stencil
 
     rng = np.random.default_rng(4)
@@ -180,6 +181,7 @@ def bench_batched(args):
     seeds = [(4 << 32) + i for i in range(count)]
     st = Stencil(5e-3, 1, 1)
     small = [i for i in range(count) if sides[i] <= 256]
+    large = [i for i in range(count) if sides[i] > 256]
 
     def batched(idx, steps):
         ens = Ensemble([fields[i] for i in idx], st, [models[i] for i in idx], [seeds[i] for i in idx], precision=32)
@@ -204,6 +206,7 @@ def bench_batched(args):
     t_small, _ = batched(small, steps)
     seq_steps = max(32, steps // 10)
     t_seq_small = sequential(small, seq_steps) * steps / seq_steps
+    t_seq_all = t_seq_small + sequential(large, seq_steps) * steps / seq_steps
     sites_all = int(sum(s * s for s in sides))
     sites_small = int(sum(sides[i] ** 2 for i in small))
     value = sites_all * steps / t_all
@@ -213,6 +216,7 @@ def bench_batched(args):
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded numpy)",
         "config": {"workload": f"C4 batched ensemble of {count} realisations", "sites": sites_all},
         "batched_vs_plain_le256": t_seq_small / t_small,
+        "batched_vs_plain_all": t_seq_all / t_all,
         "plain_sequential_le256_s_extrapolated_from": seq_steps,
         "batched_le256_site_updates_per_s": sites_small * steps / t_small,
     }
