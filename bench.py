@@ -35,6 +35,16 @@ sys.path.insert(0, REPO)
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_PROFILE = {"c3": "step_c3_fp32_4096", "c3x8192": "step_c3_fp32_8192"}
 METRIC = "site-updates/s, % HBM peak; batched vs plain; surrogate speedup at ≤2% S(k) err"
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def surrogate_flops(h, w):
+    """Algorithmic MFMA FLOPs of one inference step of the flux-form U-Net at h x w:
+    18 h w cin cout per 3x3 convolution and 2 h w cin cout per 1x1 (c1 counted with its 5
+    real input channels); the FP32 head (32 -> 2, on the vector ALUs) is not included."""
+    layers = [(1, 5, 32, 9), (2, 32, 64, 9), (4, 64, 128, 9), (4, 128, 64, 1), (2, 64, 64, 9), (2, 64, 32, 1),
+              (1, 32, 32, 9)]  # (down-sampling factor, cin, cout, taps)
+    return sum(2 * taps * (h // f) * (w // f) * cin * cout for f, cin, cout, taps in layers)
 
 WORKLOADS = {
     "c1": dict(n=256, precision=64, seed=1, stencil=(1e-3, 1, 1), model=(-0.25, 0, 0.25, 1, 0, 0, 0),
@@ -210,6 +220,7 @@ def bench_batched(args):
     sites_all = int(sum(s * s for s in sides))
     sites_small = int(sum(sides[i] ** 2 for i in small))
     value = sites_all * steps / t_all
+    achieved = 8 * sites_all * steps / t_all / 1e9
     return {
         "metric": METRIC, "value": value, "unit": "site-updates/s", "n_gpus": 1, "steps": steps,
         "warmup": args.warmup, "ms_per_step": t_all * 1e3 / steps, "higher_is_better": True, "scaling": "weak",
@@ -219,6 +230,9 @@ def bench_batched(args):
         "batched_vs_plain_all": t_seq_all / t_all,
         "plain_sequential_le256_s_extrapolated_from": seq_steps,
         "batched_le256_site_updates_per_s": sites_small * steps / t_small,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "note": "8 B per site-update over the wall clock of the graph-replayed batched steps"},
     }
 
 
@@ -295,6 +309,7 @@ def bench_surrogate(args):
     sites = side * side
     k_e, k_h = SG.mean_wavenumber(f_e), SG.mean_wavenumber(f_h)
     s = h.stats
+    tflops = surrogate_flops(side, side) * coarse / (dev_ms / 1e3) / 1e12
     return {
         "metric": METRIC, "value": sites * delta * coarse / t_hybrid, "unit": "site-updates/s (Euler-equivalent)",
         "n_gpus": 1, "steps": coarse, "warmup": args.warmup, "ms_per_step": t_hybrid * 1e3 / coarse,
@@ -310,6 +325,10 @@ def bench_surrogate(args):
         "k_mean_euler": k_e, "k_mean_rel_err": k_h / k_e - 1,
         "l2_rel_err_vs_euler": float(np.linalg.norm(f_h - f_e) / np.linalg.norm(f_e)),
         "mass_drift": float(f_h.sum() - phi.astype(np.float32).sum()),
+        "roofline": {"bound": "mfma", "achieved": tflops, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": tflops / MFMA_BF16_PEAK_TFLOPS, "traffic": None,
+                     "note": f"{surrogate_flops(side, side) / 1e9:.2f} GFLOP per inference step (five launches), "
+                             "HIP events over the surrogate-only rollout"},
     }
 
 
