@@ -90,8 +90,8 @@ StepCoef<T> make_coef(const amb_stencil& st, const amb_model& m)
     // Noise amplitude sqrt(2T) * sqrt(1/(dt dx dy)) (src/parameters.h:50,86).
     const double amp = std::sqrt(2.0 * m.T) * std::sqrt(1.0 / (st.dt * st.dx * st.dy));
     for (int j = 0; j < 5; ++j) {
-        k.nxw[j] = (T)(amp * idx * kD8[j]);
-        k.nyw[j] = (T)(amp * idy * kD8[j]);
+        k.dxw[j] = (T)(idx * kD8[j]);
+        k.dyw[j] = (T)(idy * kD8[j]);
     }
     k.dt = (T)st.dt;
     k.amp = (T)amp;

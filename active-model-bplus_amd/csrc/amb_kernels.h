@@ -17,7 +17,7 @@ struct StepCoef {
     T px[9], py[9];        // D8oD8 weights (index 1..8) times -1/dx^2, -1/dy^2
     T lxx[3], lyy[3];      // G3xoG3x row weights times -1/dx^2; lyy[0] = -1/dy^2
     T zx, zy;              // 1/dx, 1/dy (divergence of the circulating current)
-    T nxw[5], nyw[5];      // D8 weights (index 1..4) times amp/dx, amp/dy (noise)
+    T dxw[5], dyw[5];      // D8 weights (index 1..4) times 1/dx, 1/dy (currents through D8)
     T dt;
     T amp;                 // sqrt(2T) sqrt(1/(dt dx dy))
 };

@@ -124,6 +124,28 @@ __device__ __forceinline__ void mu_column(const T* s_phi, T* s_mup, const StepCo
     }
 }
 
+// Four consecutive LDS words at a 16-byte aligned address (32 bytes for double) as one
+// or two 128-bit accesses.
+__device__ __forceinline__ void lds_load4(const float* p, float v[4])
+{
+    const float4 q = *reinterpret_cast<const float4*>(p);
+    v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+}
+__device__ __forceinline__ void lds_load4(const double* p, double v[4])
+{
+    const double2 a = *reinterpret_cast<const double2*>(p), b = *reinterpret_cast<const double2*>(p + 2);
+    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+}
+__device__ __forceinline__ void lds_store4(float* p, const float v[4])
+{
+    *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ void lds_store4(double* p, const double v[4])
+{
+    *reinterpret_cast<double2*>(p) = make_double2(v[0], v[1]);
+    *reinterpret_cast<double2*>(p + 2) = make_double2(v[2], v[3]);
+}
+
 // ---- fused step --------------------------------------------------------------
 
 template <typename T, int TW, int TH, int NT, int NOISE>
@@ -138,8 +160,6 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
     T* const s_mua = sm + L::off_mua;
     T* const s_qx = sm + L::off_qx;
     T* const s_qy = sm + L::off_qy;
-    T* const s_xix = sm;                 // aliases s_phi after stage A
-    T* const s_xiy = sm + L::XW * L::XH;
 
     const int tid = threadIdx.x;
     constexpr int RG = NT / TW;        // row groups of stage B
@@ -261,13 +281,63 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
     }
     __syncthreads();
 
-    // ---- stage N: noise normals (overwrites the phi buffer)
+    // ---- stage N (noisy steps): the currents through D8, J = amp xi - (1/dx) D8 mu_p,
+    // each formed as soon as its normal is drawn and written over the phi buffer: J_x on
+    // rows [0,TH), cols [-4,TW+4); J_y on rows [-4,TH+4), cols [0,TW).  Stage B takes
+    // their D8 divergence, the passive and the noise term in one.  Every path below
+    // evaluates J in the same order (amp xi, then j = 1..4), so all give the same bits.
     bool noisy = NOISE == kNoisePhilox || NOISE == kNoiseBuffer;
     if constexpr (NOISE == kNoiseRuntime) noisy = K.amp != T(0);
+    T* const s_jx = sm;  // aliases s_phi after stage A
+    T* const s_jy = sm + L::XW * L::XH;
+    auto mup = [&](int r, int col) -> const T* { return s_mup + (r + 8) * L::MW + col + 8; };
+    // x-currents at row r, columns c0 .. c0+3 (c0 a multiple of 4, so the mu_p row and the
+    // J_x row are read and written in aligned fours)
+    auto emit_x4 = [&](int r, int c0, const T* z) {
+        T m[12], jv[4];
+        lds_load4(mup(r, c0 - 4), m);
+        lds_load4(mup(r, c0), m + 4);
+        lds_load4(mup(r, c0 + 4), m + 8);
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+            T acc = K.amp * z[l];
+#pragma unroll
+            for (int j = 1; j <= 4; ++j) acc = fma(-K.dxw[j], m[4 + l + j] - m[4 + l - j], acc);
+            jv[l] = acc;
+        }
+        lds_store4(s_jx + r * L::XW + c0 + 4, jv);
+    };
+    // y-currents at row r (from -4), columns c0 .. c0+3
+    auto emit_y4 = [&](int r, int c0, const T* z) {
+        T acc[4];
+#pragma unroll
+        for (int l = 0; l < 4; ++l) acc[l] = K.amp * z[l];
+#pragma unroll
+        for (int j = 1; j <= 4; ++j) {
+            T up[4], dn[4];
+            lds_load4(mup(r + j, c0), up);
+            lds_load4(mup(r - j, c0), dn);
+#pragma unroll
+            for (int l = 0; l < 4; ++l) acc[l] = fma(-K.dyw[j], up[l] - dn[l], acc[l]);
+        }
+        lds_store4(s_jy + (r + 4) * L::YW + c0, acc);
+    };
+    auto jx1 = [&](int r, int col, T z) {
+        const T* m = mup(r, col);
+        T acc = K.amp * z;
+#pragma unroll
+        for (int j = 1; j <= 4; ++j) acc = fma(-K.dxw[j], m[j] - m[-j], acc);
+        return acc;
+    };
+    auto jy1 = [&](int r, int col, T z) {
+        T acc = K.amp * z;
+#pragma unroll
+        for (int j = 1; j <= 4; ++j) acc = fma(-K.dyw[j], *mup(r + j, col) - *mup(r - j, col), acc);
+        return acc;
+    };
     if (NOISE == kNoisePhilox || (NOISE == kNoiseRuntime && noisy)) {
-        // One flat list of Philox calls: xi_x groups (rows [0,TH), tile-relative column
-        // groups [-1, TW/4]) then xi_y groups (rows [-4,TH+4), groups [0, TW/4)).  Each
-        // thread runs PER independent calls, unrolled so their multiply chains overlap.
+        // Philox items: xi_x groups (rows [0,TH), tile-relative column groups [-1, TW/4])
+        // and xi_y groups (rows [-4,TH+4), groups [0, TW/4)), four normals per call.
         constexpr int GXN = TW / 4 + 2, GYN = TW / 4;
         constexpr int NXI = GXN * TH, NYI = GYN * L::YH, NITEMS = NXI + NYI;
         constexpr int PERX = (NXI + NT - 1) / NT, PERY = (NYI + NT - 1) / NT;
@@ -283,8 +353,7 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
                     const int r = i / GXN, gl = i - r * GXN;
                     T z[4];
                     normals4<T>((uint32_t)((x0 >> 2) + gl - 1), (uint32_t)(y0 + r), step, 1, key0, key1, z);
-                    T* dst = s_xix + r * L::XW + 4 * gl;
-                    dst[0] = z[0]; dst[1] = z[1]; dst[2] = z[2]; dst[3] = z[3];
+                    emit_x4(r, 4 * gl - 4, z);
                 }
             }
             // xi_y: rows [-4, TH+4), column groups x0/4 .. x0/4 + TW/4 - 1
@@ -296,8 +365,7 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
                     const int rr = i / GYN, gl = i - rr * GYN;
                     T z[4];
                     normals4<T>((uint32_t)((x0 >> 2) + gl), (uint32_t)(y0 + rr - 4), step, 0, key0, key1, z);
-                    T* dst = s_xiy + rr * L::YW + 4 * gl;
-                    dst[0] = z[0]; dst[1] = z[1]; dst[2] = z[2]; dst[3] = z[3];
+                    emit_y4(rr - 4, 4 * gl, z);
                 }
             }
         } else if (nx4) {
@@ -307,26 +375,28 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
                 const int i0 = tid + u * NT;
                 const int i = i0 < NITEMS ? i0 : NITEMS - 1;
                 uint32_t group, grow;
-                int comp;
-                T* dst;
+                int comp, r, c0;
                 if (i < NXI) {
-                    const int r = i / GXN, gl = i - r * GXN - 1;
+                    r = i / GXN;
+                    const int gl = i - r * GXN - 1;
                     grow = (uint32_t)wrap_index(y0 + r, ny);
                     group = (uint32_t)(wrap_index(x0 + 4 * gl, nx) >> 2);
                     comp = 1;
-                    dst = s_xix + r * L::XW + 4 * (gl + 1);
+                    c0 = 4 * gl;
                 } else {
                     const int j = i - NXI;
                     const int rr = j / GYN, gl = j - rr * GYN;
                     grow = (uint32_t)wrap_index(y0 + rr - 4, ny);
                     group = (uint32_t)((x0 >> 2) + gl);
                     comp = 0;
-                    dst = s_xiy + rr * L::YW + 4 * gl;
+                    r = rr - 4;
+                    c0 = 4 * gl;
                 }
                 T z[4];
                 normals4<T>(group, grow, step, comp, key0, key1, z);
                 if (i0 < NITEMS) {
-                    dst[0] = z[0]; dst[1] = z[1]; dst[2] = z[2]; dst[3] = z[3];
+                    if (comp == 1) emit_x4(r, c0, z);
+                    else emit_y4(r, c0, z);
                 }
             }
         } else {
@@ -335,17 +405,18 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
                 if (i < NXI) {
                     const int r = i / GXN, gl = i - r * GXN - 1;
                     const int grow = wrap_index(y0 + r, ny);
-                    T* dst = s_xix + r * L::XW + 4 * (gl + 1);
+                    T* dst = s_jx + r * L::XW + 4 * (gl + 1);
                     for (int l = 0; l < 4; ++l)
-                        dst[l] = normal1<T>(grow, wrap_index(x0 + 4 * gl + l, nx), step, 1, key0, key1);
+                        dst[l] = jx1(r, 4 * gl + l,
+                                     normal1<T>(grow, wrap_index(x0 + 4 * gl + l, nx), step, 1, key0, key1));
                 } else {
                     const int j = i - NXI;
                     const int rr = j / GYN, gl = j - rr * GYN;
                     T z[4];
                     normals4<T>((uint32_t)((x0 >> 2) + gl), (uint32_t)wrap_index(y0 + rr - 4, ny), step, 0, key0,
                                 key1, z);
-                    T* dst = s_xiy + rr * L::YW + 4 * gl;
-                    dst[0] = z[0]; dst[1] = z[1]; dst[2] = z[2]; dst[3] = z[3];
+                    T* dst = s_jy + rr * L::YW + 4 * gl;
+                    for (int l = 0; l < 4; ++l) dst[l] = jy1(rr - 4, 4 * gl + l, z[l]);
                 }
             }
         }
@@ -353,12 +424,12 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
     } else if (NOISE == kNoiseBuffer) {
         const size_t n = (size_t)ny * nx;
         for (int i = tid; i < L::XW * L::XH; i += NT) {
-            const int r = i / L::XW, c = i - r * L::XW;
-            s_xix[i] = (T)noise[n + (size_t)wrap_index(y0 + r, ny) * nx + wrap_index(x0 + c - 4, nx)];
+            const int r = i / L::XW, col = i - r * L::XW - 4;
+            s_jx[i] = jx1(r, col, (T)noise[n + (size_t)wrap_index(y0 + r, ny) * nx + wrap_index(x0 + col, nx)]);
         }
         for (int i = tid; i < L::YW * L::YH; i += NT) {
-            const int r = i / L::YW, c = i - r * L::YW;
-            s_xiy[i] = (T)noise[(size_t)wrap_index(y0 + r - 4, ny) * nx + wrap_index(x0 + c, nx)];
+            const int rr = i / L::YW, col = i - rr * L::YW;
+            s_jy[i] = jy1(rr - 4, col, (T)noise[(size_t)wrap_index(y0 + rr - 4, ny) * nx + wrap_index(x0 + col, nx)]);
         }
         __syncthreads();
     }
@@ -366,11 +437,19 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
     // ---- stage B: one column strip of R output rows per thread.  Row-wise partial
     // sums of the separable stencils are formed once per row and shared by the
     // output rows that need them.
+    // With noise the D8 part is the divergence of J; without, D8oD8 of mu_p as one 17-tap
+    // stencil per axis (cheaper when there is no noise current to form).
+    constexpr bool kAlwaysNoisy = NOISE == kNoisePhilox || NOISE == kNoiseBuffer;
+    constexpr bool kNeverNoisy = NOISE == kNoiseNone;
     const int c = cx, r0 = rg * R;
     T* orow = out + (size_t)(y0 + r0) * nx + (x0 + c);
-    T mcol[R + 16];  // mu_p at rows r0-8 .. r0+R+7 of this column
+    T mcol[kAlwaysNoisy ? 1 : R + 16];  // mu_p at rows r0-8 .. r0+R+7 of this column
+    if constexpr (!kAlwaysNoisy) {
+        if (!noisy) {
 #pragma unroll
-    for (int k = 0; k < R + 16; ++k) mcol[k] = s_mup[(r0 + k) * L::MW + c + 8];
+            for (int k = 0; k < R + 16; ++k) mcol[k] = s_mup[(r0 + k) * L::MW + c + 8];
+        }
+    }
     T d2x[R + 4], sx[R + 4];  // mu_a rows r0-2 .. r0+R+1
 #pragma unroll
     for (int k = 0; k < R + 4; ++k) {
@@ -387,21 +466,41 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
         hx[k] = Qx[1] - Qx[-1];
         sy[k] = K.gyc * Qy[0] + K.gye * (Qy[-1] + Qy[1]);
     }
-    T ycol[NOISE == kNoiseNone ? 1 : R + 8];  // xi_y at rows r0-4 .. r0+R+3
-    if (noisy) {
+    T ycol[kNeverNoisy ? 1 : R + 8];  // J_y at rows r0-4 .. r0+R+3 of this column
+    if constexpr (!kNeverNoisy) {
+        if (noisy) {
 #pragma unroll
-        for (int k = 0; k < R + 8; ++k) ycol[k] = s_xiy[(r0 + k) * L::YW + c];
+            for (int k = 0; k < R + 8; ++k) ycol[k] = s_jy[(r0 + k) * L::YW + c];
+        }
     }
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const int r = r0 + k;
-        // passive: -(1/dx^2) D8oD8_x mu_p - (1/dy^2) D8oD8_y mu_p
-        const T* m = s_mup + (r + 8) * L::MW + c + 8;
-        T acc = K.p0 * mcol[k + 8];
+        T acc = T(0);
+        if constexpr (!kNeverNoisy) {
+            if (noisy) {
+                // passive and noise: (1/dx) D8_x J_x + (1/dy) D8_y J_y
+                const T* X = s_jx + r * L::XW + c + 4;
+                acc = K.dxw[1] * (X[1] - X[-1]);
+                acc = fma(K.dyw[1], ycol[k + 5] - ycol[k + 3], acc);
 #pragma unroll
-        for (int j = 1; j <= 8; ++j) {
-            acc = fma(K.px[j], m[j] + m[-j], acc);
-            acc = fma(K.py[j], mcol[k + 8 + j] + mcol[k + 8 - j], acc);
+                for (int j = 2; j <= 4; ++j) {
+                    acc = fma(K.dxw[j], X[j] - X[-j], acc);
+                    acc = fma(K.dyw[j], ycol[k + 4 + j] - ycol[k + 4 - j], acc);
+                }
+            }
+        }
+        if constexpr (!kAlwaysNoisy) {
+            if (!noisy) {
+                // passive: -(1/dx^2) D8oD8_x mu_p - (1/dy^2) D8oD8_y mu_p
+                const T* m = s_mup + (r + 8) * L::MW + c + 8;
+                acc = K.p0 * mcol[k + 8];
+#pragma unroll
+                for (int j = 1; j <= 8; ++j) {
+                    acc = fma(K.px[j], m[j] + m[-j], acc);
+                    acc = fma(K.py[j], mcol[k + 8 + j] + mcol[k + 8 - j], acc);
+                }
+            }
         }
         // lambda: -(1/dx^2) G3xoG3x mu_a - (1/dy^2) G3yoG3y mu_a
         acc = fma(K.lxx[0], d2x[k + 2], acc);
@@ -412,14 +511,6 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
         acc = fma(K.gxc, hx[k + 1], acc);
         acc = fma(K.gxe, hx[k] + hx[k + 2], acc);
         acc += sy[k + 2] - sy[k];
-        if (noisy) {
-            const T* X = s_xix + r * L::XW + c + 4;
-#pragma unroll
-            for (int j = 1; j <= 4; ++j) {
-                acc = fma(K.nxw[j], X[j] - X[-j], acc);
-                acc = fma(K.nyw[j], ycol[k + 4 + j] - ycol[k + 4 - j], acc);
-            }
-        }
         const T v = fma(-K.dt, acc, phi_c[k]);
         if (interior || (y0 + r < ny && x0 + c < nx)) *orow = v;
         orow += nx;
