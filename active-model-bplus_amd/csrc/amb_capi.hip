@@ -118,17 +118,27 @@ int validate_params(int64_t ny, int64_t nx, const amb_stencil* st, const amb_mod
 
 int grid_tiles(int64_t n, int t) { return (int)((n + t - 1) / t); }
 
-// Default tile: the first shape (fastest first) that still gives >= 4 workgroups per
-// CU of a 256-CU device; small fields fall back to the shape with the most tiles.
-int default_tile(int precision, int64_t ny, int64_t nx)
+// Default tile: in table order (fastest per site first, measured at 4096^2), the first
+// shape whose tiles fill the device's resident workgroups either for at least 8 rounds or
+// in whole-or-mostly-full rounds (a last round under half full leaves most CUs idle for a
+// round); fields too small for one round take the shape with the most tiles.
+int default_tile(int precision, int64_t ny, int64_t nx, int noise)
 {
+    int device = 0, cus = 256;
+    if (hipGetDevice(&device) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1) {
+        (void)hipGetLastError();
+        cus = 256;
+    }
     const int n = tile_count(precision);
     int most = 0;
     int64_t most_tiles = 0;
     for (int i = 0; i < n; ++i) {
         const TileShape t = tile_shape(precision, i);
         const int64_t tiles = (int64_t)grid_tiles(nx, t.tw) * grid_tiles(ny, t.th);
-        if (tiles >= 1024) return i;
+        const int64_t slots = (int64_t)cus * tile_occupancy(precision, i, noise);
+        const int64_t full = tiles / slots, rest = tiles % slots;
+        if (full >= 8 || (full >= 1 && (rest == 0 || 2 * rest >= slots))) return i;
         if (tiles > most_tiles) {
             most_tiles = tiles;
             most = i;
@@ -470,7 +480,6 @@ int amb_create(const double* field, int64_t ny, int64_t nx, const amb_stencil* s
     h->precision = opt.precision;
     h->esize = opt.precision == AMB_F32 ? 4 : 8;
     h->timestep = opt.timestep;
-    h->tile = opt.tile > 0 ? opt.tile - 1 : default_tile(opt.precision, ny, nx);
     const size_t n = (size_t)(ny * nx);
     hipError_t e;
 #define AMB_CHECK(call)                                  \
@@ -484,6 +493,8 @@ int amb_create(const double* field, int64_t ny, int64_t nx, const amb_stencil* s
     } while (0)
     AMB_CHECK(hipGetDevice(&h->device));
     AMB_CHECK(prepare_step_kernels());
+    h->tile = opt.tile > 0 ? opt.tile - 1
+                           : default_tile(opt.precision, ny, nx, model->T > 0.0 ? kNoisePhilox : kNoiseNone);
     AMB_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     AMB_CHECK(hipMalloc(&h->buf[0], n * h->esize));
     AMB_CHECK(hipMalloc(&h->buf[1], n * h->esize));

@@ -58,6 +58,7 @@ TileShape tile_shape(int precision, int index);
 hipError_t launch_step_f32(const StepArgs<float>& a, int tile, int noise, hipStream_t s);
 hipError_t launch_step_f64(const StepArgs<double>& a, int tile, int noise, hipStream_t s);
 hipError_t prepare_step_kernels();  // raises the dynamic-LDS limit of large tiles
+int tile_occupancy(int precision, int tile, int noise);  // resident workgroups per CU (>= 1)
 
 // Adds `n` to the device step counter (a one-lane kernel, graph-capturable).
 hipError_t launch_advance(long long* step, long long n, hipStream_t s);

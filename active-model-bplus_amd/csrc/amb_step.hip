@@ -95,6 +95,39 @@ hipError_t launch_step_f64(const StepArgs<double>& a, int tile, int noise, hipSt
     }
 }
 
+// Resident workgroups per CU of one step kernel (registers and LDS), at least 1.
+template <typename T, int TW, int TH, int NT>
+static int occupancy_of(int noise)
+{
+    using L = Lds<T, TW, TH, NT>;
+    const void* f = noise == kNoiseNone     ? (const void*)step_kernel<T, TW, TH, NT, kNoiseNone>
+                    : noise == kNoisePhilox ? (const void*)step_kernel<T, TW, TH, NT, kNoisePhilox>
+                                            : (const void*)step_kernel<T, TW, TH, NT, kNoiseBuffer>;
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, NT, L::bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    return n > 0 ? n : 1;
+}
+
+#define AMB_OCC_F32(i, tw, th, nt) case i: return occupancy_of<float, tw, th, nt>(noise);
+#define AMB_OCC_F64(i, tw, th, nt) case i: return occupancy_of<double, tw, th, nt>(noise);
+
+int tile_occupancy(int precision, int tile, int noise)
+{
+    if (precision == 32) {
+        switch (tile) {
+            AMB_F32_TILES(AMB_OCC_F32)
+        default: return 1;
+        }
+    }
+    switch (tile) {
+        AMB_F64_TILES(AMB_OCC_F64)
+    default: return 1;
+    }
+}
+
 template <typename T, int TW, int TH, int NT>
 static hipError_t raise_lds_limit()
 {
