@@ -47,7 +47,7 @@ struct TileShape {
 // Compiled tile configurations (index = autotuner choice): X(index, TW, TH, NT).
 // Index 0 is the fastest measured on MI355X at 4096^2; the default picks the first
 // entry that still gives at least four workgroups per CU.
-#define AMB_F32_TILES(X) X(0, 32, 24, 256) X(1, 32, 32, 256) X(2, 64, 32, 512) X(3, 64, 16, 256) X(4, 64, 64, 1024) X(5, 64, 32, 256) X(6, 32, 16, 256)
+#define AMB_F32_TILES(X) X(0, 32, 48, 512) X(1, 32, 24, 256) X(2, 32, 32, 256) X(3, 64, 32, 512) X(4, 64, 16, 256) X(5, 32, 16, 256)
 #define AMB_F64_TILES(X) X(0, 32, 32, 256) X(1, 64, 16, 256) X(2, 32, 16, 256) X(3, 32, 32, 512)
 
 // Tile shapes compiled per precision (index = autotuner choice).
