@@ -62,7 +62,7 @@ __device__ __forceinline__ void box_muller(uint32_t wa, uint32_t wb, double& z0,
     const double u2 = (double)wb * 0x1p-32;
     const double r = sqrt(-2.0 * log(u1));
     double s, c;
-    sincos(2.0 * M_PI * u2, &s, &c);
+    sincospi(2.0 * u2, &s, &c);  // sin and cos of 2 pi u2 (2 u2 is exact; no range reduction by pi)
     z0 = r * c;
     z1 = r * s;
 }
