@@ -88,6 +88,8 @@ class ActiveModelBPlus:
         a = self.alpha
         if derivative == 1:
             return np.exp(a * phi)
+        if derivative != 0:
+            raise ValueError("derivative must be 0 or 1")
         if abs(a * np.max(np.abs(phi), initial=0.0)) < 1e-8:
             return phi + a * phi ** 2 / 2  # series; exact to round-off in this range
         return np.expm1(a * phi) / a

@@ -155,3 +155,5 @@ def test_argument_errors():
         AD.ActiveModelBPlus(0, 0).droplet(10, domain_size=5)
     with pytest.raises(ValueError):
         AD.ActiveModelBPlus(0, 0).surface_tension("drops")
+    with pytest.raises(ValueError):
+        AD.ActiveModelBPlus(0, 0).pseudodensity(0.5, derivative=2)
