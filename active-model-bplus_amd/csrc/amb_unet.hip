@@ -586,7 +586,7 @@ __global__ void head_fwd_kernel(const bf16* __restrict__ d1, const float* __rest
 // up-sampled u1 plus the skip e1, the head (3x3, 32 -> 2, FP32 weights) and the flux
 // update phi_out = phi - div J.  The block stages s1 = up(u1) + e1 over rows
 // y0-3 .. y0+TYO+2 and cols x0-3 .. x0+17 in LDS, computes d1 on rows y0-2 .. y0+TYO
-// and cols x0-2 .. x0+16 with MFMA (two overlapping 16-wide column tiles per row) into
+// and cols x0-2 .. x0+16 with MFMA (16-pixel tiles of that region in row-major order) into
 // LDS, J on rows y0-1 .. y0+TYO-1 and cols x0-1 .. x0+15, then the update of its
 // TYO x 16 pixels.  J_out (nullable) receives the tile's J.  Eight waves.
 template <int TYO>
@@ -600,8 +600,10 @@ __global__ __launch_bounds__(512) void c7_head_flux_kernel(const bf16* __restric
     constexpr int SW = 21, DW = 19, JW = 17;                // staged, d1 and J region widths
     constexpr int SH = TYO + 6, DH = TYO + 3, JH = TYO + 1;  // their heights
     constexpr int NS = 9;                                  // K-steps of c7 (K = 9 x 32)
-    constexpr int ROWT = 2 * DH, MBW = (ROWT + 7) / 8;      // row tiles over 8 waves
+    constexpr int NP = DH * DW, MT = (NP + 15) / 16;        // d1 pixels, their 16-pixel tiles
+    constexpr int MBW = (MT + 7) / 8;                      // tiles per wave
     static_assert(JH * JW * 2 * 4 <= SH * SW * SP * 2, "J must fit in the staging buffer");
+    static_assert(JH * JW <= 256, "one head pixel per thread of each half block");
     __shared__ __align__(16) bf16 sin[SH * SW * SP];  // later reused for J
     __shared__ __align__(16) bf16 sd[DH * DW * SP];
     float* sj = reinterpret_cast<float*>(sin);
@@ -629,12 +631,18 @@ __global__ __launch_bounds__(512) void c7_head_flux_kernel(const bf16* __restric
         if (i < ITEMS) *reinterpret_cast<bf16x8*>(sin + (i >> 2) * SP + 8 * (i & 3)) = sv[k];
     }
 
-    // c7: row tile rt = (row i in 0..DH-1, column tile ct): d1 row y0-2+i, columns
-    // x0-2+16ct' .. with ct' = 0 at x0-2 and ct' = 1 at x0+1 (overlapping)
-    const bf16x8 zero = {};
+    // c7 over the d1 region (rows y0-2 .. y0+TYO, columns x0-2 .. x0+16) taken row-major as
+    // NP pixels: tile mt holds pixels 16 mt .. 16 mt + 15, so no tile straddles computed
+    // pixels that are not needed.  K-step st is tap st (32 channels), lane quarter q reads
+    // channels 8q .. 8q+7; lanes past the last pixel read a valid pixel and are not stored.
     f32x4 acc[MBW][2];
+    int abase[MBW];
 #pragma unroll
-    for (int m = 0; m < MBW; ++m) acc[m][0] = acc[m][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int m = 0; m < MBW; ++m) {
+        acc[m][0] = acc[m][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+        const int p = min(16 * (wave + 8 * m) + l16, NP - 1), i = p / DW, c = p - i * DW;
+        abase[m] = ((i + 1) * SW + c + 1) * SP + 8 * q;  // staged coordinates are d1 coordinates + 1
+    }
     auto load_b = [&](int st, bf16x8* bb) {
         const int k0 = 32 * st + 8 * q, tap = k0 / 32, ci = k0 - 32 * tap;
         bb[0] = load8(w7 + ((size_t)tap * 32 + l16) * 32 + ci);
@@ -648,67 +656,61 @@ __global__ __launch_bounds__(512) void c7_head_flux_kernel(const bf16* __restric
 #pragma unroll
     for (int st = 0; st < NS; ++st) {
         if (st + PF - 1 < NS) load_b(st + PF - 1, bq[(st + PF - 1) % PF]);
-        const int k0 = 32 * st + 8 * q, tap = k0 / 32, ci = k0 - 32 * tap;
-        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+        const int dy = st / 3 - 1, dx = st % 3 - 1;
 #pragma unroll
         for (int m = 0; m < MBW; ++m) {
-            const int rt = wave + 8 * m;
-            if (rt < ROWT) {
-                const int i = rt >> 1, c0 = (rt & 1) ? 3 : 0;  // d1 row, first column (d1 coordinates)
-                // staged coordinates are d1 coordinates + 1
-                const bf16x8 a =
-                    *reinterpret_cast<const bf16x8*>(sin + ((i + 1 + dy) * SW + c0 + l16 + 1 + dx) * SP + ci);
+            if (wave + 8 * m < MT) {
+                const bf16x8 a = *reinterpret_cast<const bf16x8*>(sin + abase[m] + (dy * SW + dx) * SP);
                 acc[m][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bq[st % PF][0], acc[m][0], 0, 0, 0);
                 acc[m][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bq[st % PF][1], acc[m][1], 0, 0, 0);
             }
         }
     }
+    // D fragment element e of lane (q, l16) is pixel 16 mt + 4q + e, channel 16 j + l16
 #pragma unroll
     for (int m = 0; m < MBW; ++m) {
-        const int rt = wave + 8 * m;
-        if (rt < ROWT) {
-            const int i = rt >> 1, c0 = (rt & 1) ? 3 : 0;
+        const int mt = wave + 8 * m;
+        if (mt < MT) {
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
                 const int co = 16 * j + l16;
                 const float bco = b7[co];
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    const int px = 4 * q + e, col = c0 + px;
-                    // the second column tile writes only the columns the first lacks
-                    if (c0 == 0 || px >= 13) sd[(i * DW + col) * SP + co] = f2bf(gelu(acc[m][j][e] + bco));
+                    const int p = 16 * mt + 4 * q + e;
+                    if (p < NP) sd[p * SP + co] = f2bf(gelu(acc[m][j][e] + bco));
                 }
             }
         }
     }
     __syncthreads();
 
-    // head: J at d1 coordinates 1 .. JH (rows y0-1 .. y0+TYO-1), cols 1 .. 17; the
-    // weights are uniform across the block and read as scalars; summation order as in
-    // head_fwd_kernel
-    if (threadIdx.x < JH * JW) {
-        const int i = threadIdx.x, r = i / JW + 1, c = i % JW + 1;
-        float j0[3] = {bh[0], 0.f, 0.f}, j1[3] = {bh[1], 0.f, 0.f};
+    // head: J at d1 coordinates 1 .. JH (rows y0-1 .. y0+TYO-1), cols 1 .. 17; the first
+    // half of the block computes J's component 0, the second half component 1, one pixel
+    // per thread.  The component is uniform across each wave, so the weights are read as
+    // scalars; summation order as in head_fwd_kernel (one chain per tap column).
+    {
+        const int ch = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), i = threadIdx.x & 255;
+        if (i < JH * JW) {
+            const int r = i / JW + 1, c = i % JW + 1;
+            float jc[3] = {bh[ch], 0.f, 0.f};
 #pragma unroll
-        for (int tap = 0; tap < 9; ++tap) {
-            const bf16* sp = sd + ((r + tap / 3 - 1) * DW + (c + tap % 3 - 1)) * SP;
-            const float* w0 = wh + tap * 64;
+            for (int tap = 0; tap < 9; ++tap) {
+                const bf16* sp = sd + ((r + tap / 3 - 1) * DW + (c + tap % 3 - 1)) * SP;
+                const float* w0 = wh + tap * 64 + 32 * ch;
 #pragma unroll
-            for (int c8 = 0; c8 < 4; ++c8) {
-                const u32x4 v = *reinterpret_cast<const u32x4*>(sp + 8 * c8);
+                for (int c8 = 0; c8 < 4; ++c8) {
+                    const u32x4 v = *reinterpret_cast<const u32x4*>(sp + 8 * c8);
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int ci = 8 * c8 + 2 * e;
-                    const float lo = lo_f(v[e]), hi = hi_f(v[e]);
-                    j0[tap % 3] = fmaf(lo, w0[ci], j0[tap % 3]);
-                    j1[tap % 3] = fmaf(lo, w0[32 + ci], j1[tap % 3]);
-                    j0[tap % 3] = fmaf(hi, w0[ci + 1], j0[tap % 3]);
-                    j1[tap % 3] = fmaf(hi, w0[33 + ci], j1[tap % 3]);
+                    for (int e = 0; e < 4; ++e) {
+                        const int ci = 8 * c8 + 2 * e;
+                        jc[tap % 3] = fmaf(lo_f(v[e]), w0[ci], jc[tap % 3]);
+                        jc[tap % 3] = fmaf(hi_f(v[e]), w0[ci + 1], jc[tap % 3]);
+                    }
                 }
             }
+            sj[2 * i + ch] = kJScale * ((jc[0] + jc[1]) + jc[2]);
         }
-        sj[2 * i] = kJScale * ((j0[0] + j0[1]) + j0[2]);
-        sj[2 * i + 1] = kJScale * ((j1[0] + j1[1]) + j1[2]);
     }
     __syncthreads();
     if (threadIdx.x < TYO * 16) {
