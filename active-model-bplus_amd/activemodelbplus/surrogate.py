@@ -357,7 +357,8 @@ def _main(argv=None):
     t.add_argument("--noise", type=float, default=0.3, help="relative training-input noise")
     t.add_argument("--val-realisations", type=int, default=8)
     f = sub.add_parser("profile", help="inference steps and training iterations on synthetic data, for a profiler")
-    f.add_argument("--steps", type=int, default=200)
+    f.add_argument("--steps", type=int, default=1000)
+    f.add_argument("--warmup", type=int, default=3000, help="untimed inference steps first (GPU clock ramp)")
     f.add_argument("--train-iterations", type=int, default=50)
     e = sub.add_parser("evaluate")
     e.add_argument("--weights", required=True)
@@ -379,7 +380,7 @@ def _main(argv=None):
         sur = Surrogate(256, 256, max_batch=1, seed=1)
         if os.path.exists(default_weights_path()):
             load_weights(sur, default_weights_path())
-        sur.advance(sim, 4, [0.5, -1.0, 0.0], DELTA_STEPS)
+        sur.advance(sim, max(a.warmup, 1), [0.5, -1.0, 0.0], DELTA_STEPS)
         ms = sur.advance(sim, a.steps, [0.5, -1.0, 0.0], DELTA_STEPS)
         log(f"inference: {ms / a.steps * 1e3:.1f} us per step (batch 1, 256^2)")
         tr = Surrogate(256, 256, max_batch=16, seed=1)
