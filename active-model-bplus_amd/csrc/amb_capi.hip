@@ -118,9 +118,10 @@ int validate_params(int64_t ny, int64_t nx, const amb_stencil* st, const amb_mod
 
 int grid_tiles(int64_t n, int t) { return (int)((n + t - 1) / t); }
 
-// Default tile: in table order (fastest per site first, measured at 4096^2), the first
-// shape whose tiles fill the device's resident workgroups either for at least 8 rounds or
-// in whole-or-mostly-full rounds (a last round under half full leaves most CUs idle for a
+// Default tile: in order of preference (fastest per site first, measured at 4096^2: the
+// table order, or AMB_F32_NOISELESS_ORDER for the noiseless FP32 kernel), the first shape
+// whose tiles fill the device's resident workgroups either for at least 8 rounds or in
+// whole-or-mostly-full rounds (a last round under half full leaves most CUs idle for a
 // round); fields too small for one round take the shape with the most tiles.
 int default_tile(int precision, int64_t ny, int64_t nx, int noise)
 {
@@ -130,10 +131,17 @@ int default_tile(int precision, int64_t ny, int64_t nx, int noise)
         (void)hipGetLastError();
         cus = 256;
     }
+    static constexpr int kNoiselessF32[] = {AMB_F32_NOISELESS_ORDER};
+#define AMB_COUNT_ONE(i, tw, th, nt) +1
+    static_assert(sizeof(kNoiselessF32) / sizeof(kNoiselessF32[0]) == 0 AMB_F32_TILES(AMB_COUNT_ONE),
+                  "the noiseless FP32 order must list every FP32 tile shape");
+#undef AMB_COUNT_ONE
     const int n = tile_count(precision);
+    const bool reorder = precision == AMB_F32 && noise == kNoiseNone;
     int most = 0;
     int64_t most_tiles = 0;
-    for (int i = 0; i < n; ++i) {
+    for (int k = 0; k < n; ++k) {
+        const int i = reorder ? kNoiselessF32[k] : k;
         const TileShape t = tile_shape(precision, i);
         const int64_t tiles = (int64_t)grid_tiles(nx, t.tw) * grid_tiles(ny, t.th);
         const int64_t slots = (int64_t)cus * tile_occupancy(precision, i, noise);

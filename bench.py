@@ -14,7 +14,8 @@ launch duration from HIP events on the integrator's own stream.  ``cpu_baseline`
 times the CPU FP64 oracle (oracle/, the checker) on this host on a bounded sample.
 
 Other workloads (--workload): c1 (256^2 FP64 spinodal), c2 (1024^2 FP32 noisy
-active), c3 (default), c3f64 (4096^2 FP64), c3x8192 (the C3 model on an 8192^2 FP32
+active), c3 (default), c3t0 (the C3 model without noise: the deterministic path), c3f64
+(4096^2 FP64), c3x8192 (the C3 model on an 8192^2 FP32
 field: phi^n and phi^(n+1) together are 512 MiB, past the 256 MB MALL), c4 (batched ensemble of 512
 realisations, sides 64..512, vs sequential plain runs on the <=256^2 subset), c5
 (learned surrogate with Euler audits vs plain Euler, 256^2; --steps = coarse steps of
@@ -53,6 +54,8 @@ WORKLOADS = {
                init=("normal", -0.4, 0.01), label="C2 AMB+ noisy 1024x1024 FP32"),
     "c3": dict(n=4096, precision=32, seed=3, stencil=(5e-3, 1, 1), model=(-0.25, 0, 0.25, 1, 0.5, -1, 0.05),
                init=("normal", 0.0, 0.05), label="C3 AMB+ noisy 4096x4096 FP32 (roofline)"),
+    "c3t0": dict(n=4096, precision=32, seed=3, stencil=(5e-3, 1, 1), model=(-0.25, 0, 0.25, 1, 0.5, -1, 0.0),
+                 init=("normal", 0.0, 0.05), label="C3 model at T = 0 (noiseless) 4096x4096 FP32"),
     "c3f64": dict(n=4096, precision=64, seed=3, stencil=(5e-3, 1, 1), model=(-0.25, 0, 0.25, 1, 0.5, -1, 0.05),
                   init=("normal", 0.0, 0.05), label="C3 AMB+ noisy 4096x4096 FP64"),
     "c3x8192": dict(n=8192, precision=32, seed=3, stencil=(5e-3, 1, 1), model=(-0.25, 0, 0.25, 1, 0.5, -1, 0.05),
