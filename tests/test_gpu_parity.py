@@ -244,19 +244,34 @@ def test_fp32_against_fp64(amb, oracle):
     assert bad == 0 and abs(s - phi.sum()) / phi.size < 1e-6
 
 
-def test_tile_shapes_bit_identical(amb):
-    """Every compiled tile shape computes each site with the same arithmetic."""
+@pytest.mark.parametrize("temperature", [0.1, 0.0])
+def test_tile_shapes_bit_identical(amb, temperature):
+    """Every compiled tile shape computes each site with the same arithmetic, in the noisy
+    and in the noiseless kernel."""
     rng = np.random.default_rng(8)
     for precision in (64, 32):
         phi = -0.3 + 0.05 * rng.normal(size=(96, 160))
         out = []
         for t in range(len(amb.tile_shapes(precision))):
-            sim = make(amb, phi, (5e-3, 1, 1), (-0.25, 0, 0.25, 1, -1, -4, 0.1), seed=5,
+            sim = make(amb, phi, (5e-3, 1, 1), (-0.25, 0, 0.25, 1, -1, -4, temperature), seed=5,
                        precision=precision, tile=t + 1)
             sim.run(40)
             out.append(sim.field)
         for o in out[1:]:
             assert np.array_equal(o, out[0])
+
+
+def test_default_tile_per_noise_mode(amb):
+    """Without autotuning the tile comes from the occupancy rule over each kernel's order of
+    preference: at 4096^2 FP32 the noisy kernel takes 32x48 tiles of 512 threads, the
+    noiseless kernel (its 17-tap windows need more registers) 32x24 tiles of 256 threads;
+    a 1024^2 field takes 32x32 either way (one full round of resident workgroups)."""
+    shapes = amb.tile_shapes(32)
+    for side, temperature, want in [(4096, 0.05, (32, 48, 512)), (4096, 0.0, (32, 24, 256)),
+                                    (1024, 0.1, (32, 32, 256)), (1024, 0.0, (32, 32, 256))]:
+        sim = make(amb, np.zeros((side, side)), (5e-3, 1, 1), (-0.25, 0, 0.25, 1, 0.5, -1, temperature),
+                   precision=32)
+        assert tuple(shapes[sim.options["tile"] - 1]) == want, (side, temperature)
 
 
 def test_graph_and_direct_launch_agree(amb):
