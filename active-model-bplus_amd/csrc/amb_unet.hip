@@ -50,15 +50,9 @@ constexpr float kGeluC0 = 0.7978845608028654f, kGeluC1 = 0.044715f;
 
 __device__ __forceinline__ float bf2f(bf16 h) { return __uint_as_float((uint32_t)h << 16); }
 
-// float -> bfloat16, round to nearest even (NaN kept quiet).
-__host__ __device__ __forceinline__ bf16 f2bf(float f)
-{
-    uint32_t u;
-    std::memcpy(&u, &f, 4);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (bf16)((u >> 16) | 0x40u);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (bf16)(u >> 16);
-}
+// float -> bfloat16, round to nearest even: gfx950's v_cvt_pk_bf16_f32 (one instruction
+// for two values where the compiler pairs them).
+__device__ __forceinline__ bf16 f2bf(float f) { return __builtin_bit_cast(bf16, (__bf16)f); }
 
 // tanh(u) = 1 - 2 / (exp(2u) + 1) with the hardware 2^x (v_exp_f32) and reciprocal
 // (v_rcp_f32): saturates to +-1 for large |u|, absolute error ~1e-7 near 0, far below
@@ -387,6 +381,158 @@ __global__ __launch_bounds__(64 * WM * WN * KW) void conv_tile_kernel(const Conv
                 out[(((size_t)n * H + y0 + r) * W + x0 + 4 * q + e) * C2 + 16 * j + l16] = f2bf(d[e] + bb);
         }
     }
+}
+
+// Inference c1 and c2 in one kernel per TY x 16 tile of c2's output (half resolution).
+// The block assembles the input on the (2 TY + 6) x 38 full-resolution pixels around the
+// tile, computes c1 (3x3, 8 staged channels -> 32, GELU) with MFMA on the (2 TY + 4) x 36
+// pixels that the tile's pooled 3x3 operand covers (16-pixel tiles of that region in
+// row-major order), keeps them in LDS and writes its own 2 TY x 32 of them to e1 (c7 reads
+// e1 as its skip), pools them 2x2 into c2's (TY + 2) x 18 operand tile and computes c2
+// (3x3, 32 -> 64, GELU) into e2.  Every c1 and c2 output is formed from the same MFMA
+// operands in the same K order, with the same bias, GELU and rounding, as
+// conv_tile_kernel forms it, so e1 and e2 equal those of two separate launches bit for
+// bit.  Neighbouring blocks recompute the c1 halo (1.69x the c1 work at TY = 4), which
+// costs less than the launch it replaces.  TY waves per 16-channel group of c2, 4 groups.
+template <int TY>
+__global__ __launch_bounds__(256 * TY) void c12_kernel(const float* __restrict__ phi, const float* __restrict__ params,
+                                                       const bf16* __restrict__ w1, const float* __restrict__ b1,
+                                                       const bf16* __restrict__ w2, const float* __restrict__ b2, int H,
+                                                       int W, bf16* __restrict__ e1, bf16* __restrict__ e2)
+{
+    constexpr int NT = 256 * TY, NWAVE = NT / 64;
+    constexpr int IR = 2 * TY + 6, IC = 38, LDP1 = 16;  // input tile: 8 channels + 8 padding per pixel
+    constexpr int ER = 2 * TY + 4, EC = 36, LDE = 40;   // c1 tile: 32 channels + 8 padding
+    constexpr int NP1 = ER * EC, MT1 = (NP1 + 15) / 16, MB1 = (MT1 + NWAVE - 1) / NWAVE;
+    constexpr int PR = TY + 2, PW = 18, LDP2 = 40;      // c2's pooled operand tile
+    constexpr int NS2 = 9;                              // c2 K-steps (K = 9 x 32)
+    __shared__ __align__(16) bf16 s_in[IR * IC * LDP1];
+    __shared__ __align__(16) bf16 s_e1[NP1 * LDE];
+    __shared__ __align__(16) bf16 s_p[PR * PW * LDP2];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int q = lane >> 4, l16 = lane & 15;
+    const int H2 = H / 2, W2 = W / 2, tiles_x = W2 / 16, tiles_y = H2 / TY;
+    const int bx = blockIdx.x % tiles_x, t = blockIdx.x / tiles_x, by = t % tiles_y, n = t / tiles_y;
+    const int x0 = 16 * bx, y0 = TY * by;          // c2 tile origin (half resolution)
+    const int fy0 = 2 * y0 - 3, fx0 = 2 * x0 - 3;  // input tile origin (full resolution)
+
+    // input assembly, every gather of the thread first
+    constexpr int PER1 = (IR * IC + NT - 1) / NT;
+    const ConvIn src{nullptr, nullptr, phi, params};
+    bf16x8 sv[PER1];
+#pragma unroll
+    for (int k = 0; k < PER1; ++k) {
+        const int i = tid + NT * k;
+        if (i < IR * IC) {
+            const int ry = i / IC, rx = i - ry * IC;
+            sv[k] = load_a<kSrcInput>(src, 8, n, wrapi(fy0 + ry, H), wrapi(fx0 + rx, W), H, W, 0);
+        }
+    }
+    // weight fragments: c1's three K-steps (tap k0 / 8 of lane quarter q, zero past the
+    // 72 real K) for both 16-channel tiles, and c2's nine K-steps for this wave's group
+    const bf16x8 zero = {};
+    bf16x8 bw1[3][2], bw2[NS2];
+#pragma unroll
+    for (int st = 0; st < 3; ++st) {
+        const int k0 = 32 * st + 8 * q;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) bw1[st][j] = k0 < 72 ? load8(w1 + ((size_t)(k0 / 8) * 32 + 16 * j + l16) * 8) : zero;
+    }
+    const int wm = wave % TY, wn = wave / TY;
+#pragma unroll
+    for (int st = 0; st < NS2; ++st) bw2[st] = load8(w2 + ((size_t)st * 64 + 16 * wn + l16) * 32 + 8 * q);
+#pragma unroll
+    for (int k = 0; k < PER1; ++k) {
+        const int i = tid + NT * k;
+        if (i < IR * IC) *reinterpret_cast<bf16x8*>(s_in + i * LDP1) = sv[k];
+    }
+
+    // c1 on the covered region; lanes past its last pixel read a valid pixel and are not stored
+    f32x4 acc1[MB1][2];
+    int abase[MB1];
+#pragma unroll
+    for (int m = 0; m < MB1; ++m) {
+        acc1[m][0] = acc1[m][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+        const int p = min(16 * (wave + NWAVE * m) + l16, NP1 - 1), er = p / EC, ec = p - er * EC;
+        abase[m] = ((er + 1) * IC + ec + 1) * LDP1;  // input coordinates are c1 coordinates + 1
+    }
+    __syncthreads();
+#pragma unroll
+    for (int st = 0; st < 3; ++st) {
+        const int k0 = 32 * st + 8 * q, tap = k0 / 8;
+        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+#pragma unroll
+        for (int m = 0; m < MB1; ++m) {
+            if (wave + NWAVE * m < MT1) {
+                bf16x8 a = zero;
+                if (k0 < 72) a = *reinterpret_cast<const bf16x8*>(s_in + abase[m] + (dy * IC + dx) * LDP1);
+                acc1[m][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw1[st][0], acc1[m][0], 0, 0, 0);
+                acc1[m][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw1[st][1], acc1[m][1], 0, 0, 0);
+            }
+        }
+    }
+    // D fragment element e of lane (q, l16) is pixel 16 mt + 4q + e, channel 16 j + l16
+#pragma unroll
+    for (int m = 0; m < MB1; ++m) {
+        const int mt = wave + NWAVE * m;
+        if (mt < MT1) {
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int co = 16 * j + l16;
+                const float bco = b1[co];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int p = 16 * mt + 4 * q + e;
+                    if (p < NP1) {
+                        const bf16 g = f2bf(gelu(acc1[m][j][e] + bco));
+                        s_e1[p * LDE + co] = g;
+                        const int er = p / EC, ec = p - er * EC;
+                        if (er >= 2 && er < 2 * TY + 2 && ec >= 2 && ec < 34)
+                            e1[(((size_t)n * H + fy0 + 1 + er) * W + fx0 + 1 + ec) * 32 + co] = g;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    // 2x2 average pooling into c2's operand tile, summed and rounded as load_a<kSrcPool>
+    constexpr int ITEMS2 = PR * PW * 4, PER2 = (ITEMS2 + NT - 1) / NT;
+#pragma unroll
+    for (int k = 0; k < PER2; ++k) {
+        const int i = tid + NT * k;
+        if (i < ITEMS2) {
+            const int c4 = i & 3, pp = i >> 2, pr = pp / PW, pc = pp - pr * PW;
+            const bf16* b = s_e1 + (2 * pr * EC + 2 * pc) * LDE + 8 * c4;
+            const u32x4 v00 = *reinterpret_cast<const u32x4*>(b), v01 = *reinterpret_cast<const u32x4*>(b + LDE),
+                        v10 = *reinterpret_cast<const u32x4*>(b + EC * LDE),
+                        v11 = *reinterpret_cast<const u32x4*>(b + EC * LDE + LDE);
+            u32x4 r;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float lo = lo_f(v00[j]) + lo_f(v01[j]) + lo_f(v10[j]) + lo_f(v11[j]);
+                const float hi = hi_f(v00[j]) + hi_f(v01[j]) + hi_f(v10[j]) + hi_f(v11[j]);
+                r[j] = (uint32_t)f2bf(0.25f * lo) | ((uint32_t)f2bf(0.25f * hi) << 16);
+            }
+            *reinterpret_cast<u32x4*>(s_p + pp * LDP2 + 8 * c4) = r;
+        }
+    }
+    __syncthreads();
+
+    // c2: row wm of the tile, channels 16 wn .. 16 wn + 15
+    f32x4 acc2 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int st = 0; st < NS2; ++st) {
+        const int dy = st / 3 - 1, dx = st % 3 - 1;
+        const bf16x8 a =
+            *reinterpret_cast<const bf16x8*>(s_p + ((wm + 1 + dy) * PW + l16 + 1 + dx) * LDP2 + 8 * q);
+        acc2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw2[st], acc2, 0, 0, 0);
+    }
+    const int co = 16 * wn + l16;
+    const float bco = b2[co];
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+        e2[(((size_t)n * H2 + y0 + wm) * W2 + x0 + 4 * q + e) * 64 + co] = f2bf(gelu(acc2[e] + bco));
 }
 
 // ---- weight gradient ----------------------------------------------------------------
@@ -1245,10 +1391,11 @@ int forward_infer(amb_surrogate* s, int n, const float* phi, const float* params
     // <SRC, CIN, NB, TY, WM, WN, NB2>: small tiles, so that several blocks share a CU
     // and hide each other's memory latency
     static_assert(kCinReal <= 8, "c1 inference packs the data channels into 8");
-    SUR_CHECK((conv_tile_launch<kSrcInput, 8, 2, 4, 2, 2>(ConvIn{nullptr, nullptr, phi, params}, s->wf0_8,
-                                                           M + s->b_off[0], n, H, W, s->e1, st)));
-    SUR_CHECK((conv_tile_launch<kSrcPool, 32, 4, 1, 1, 4>(plain(s->e1), s->wf[1], M + s->b_off[1], n, H2, W2, s->e2,
-                                                           st)));
+    constexpr int kC12Rows = 4;
+    if (H2 % kC12Rows || W2 % 16) return fail(AMB_E_ARG, "surrogate grid does not fit the inference tiles");
+    hipLaunchKernelGGL(c12_kernel<kC12Rows>, dim3(n * (H2 / kC12Rows) * (W2 / 16)), dim3(256 * kC12Rows), 0, st, phi,
+                       params, s->wf0_8, M + s->b_off[0], s->wf[1], M + s->b_off[1], H, W, s->e1, s->e2);
+    SUR_CHECK(hipGetLastError());
     SUR_CHECK((conv_tile_launch<kSrcPool, 64, 8, 1, 1, 4, 4, 2>(plain(s->e2), s->wf[2], M + s->b_off[2], n, H4, W4,
                                                                  s->u2, st, s->wf[3], M + s->b_off[3])));
     SUR_CHECK((conv_tile_launch<kSrcUpAdd, 64, 4, 2, 1, 4, 2>(ConvIn{s->u2, s->e2, nullptr, nullptr}, s->wf[4],
