@@ -1376,11 +1376,11 @@ int forward(amb_surrogate* s, int n, const float* phi, const float* params, floa
     return AMB_OK;
 }
 
-// Inference: phi_out = phi - div J in five launches: c1 (input assembly); c2 (pooled
+// Inference: phi_out = phi - div J in four launches: c1 (input assembly) with c2 (pooled
 // operand); c3 (pooled operand) with c4; c5 (up-sampled operand) with c6; c7
-// (up-sampled operand) with the head and the flux update.  Pooling, up-sampling adds and the input layer are applied
-// once per pixel while staging a tile in LDS; the 1x1 layers run in the epilogue of
-// the 3x3 layer before them.  J_out (nullable) receives the current.
+// (up-sampled operand) with the head and the flux update.  Pooling, up-sampling adds and
+// the input layer are applied once per pixel while staging a tile in LDS; the 1x1 layers
+// run in the epilogue of the 3x3 layer before them.  J_out (nullable) receives the current.
 int forward_infer(amb_surrogate* s, int n, const float* phi, const float* params, float* phi_out,
                   float* J_out = nullptr)
 {

@@ -330,7 +330,7 @@ def bench_surrogate(args):
         "mass_drift": float(f_h.sum() - phi.astype(np.float32).sum()),
         "roofline": {"bound": "mfma", "achieved": tflops, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": tflops / MFMA_BF16_PEAK_TFLOPS, "traffic": None,
-                     "note": f"{surrogate_flops(side, side) / 1e9:.2f} GFLOP per inference step (five launches), "
+                     "note": f"{surrogate_flops(side, side) / 1e9:.2f} GFLOP per inference step (four launches), "
                              "HIP events over the surrogate-only rollout"},
     }
 
