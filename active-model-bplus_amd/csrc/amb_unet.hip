@@ -750,8 +750,9 @@ __global__ __launch_bounds__(512) void c7_head_flux_kernel(const bf16* __restric
     constexpr int MBW = (MT + 7) / 8;                      // tiles per wave
     static_assert(JH * JW * 2 * 4 <= SH * SW * SP * 2, "J must fit in the staging buffer");
     static_assert(JH * JW <= 256, "one head pixel per thread of each half block");
+    constexpr int SPF = 36;                            // d1 pixel stride in floats (32 channels + 4 padding)
     __shared__ __align__(16) bf16 sin[SH * SW * SP];  // later reused for J
-    __shared__ __align__(16) bf16 sd[DH * DW * SP];
+    __shared__ __align__(16) float sd[DH * DW * SPF];  // d1, bfloat16 values widened to float for the head
     float* sj = reinterpret_cast<float*>(sin);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int q = lane >> 4, l16 = lane & 15;
@@ -824,7 +825,7 @@ __global__ __launch_bounds__(512) void c7_head_flux_kernel(const bf16* __restric
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const int p = 16 * mt + 4 * q + e;
-                    if (p < NP) sd[p * SP + co] = f2bf(gelu(acc[m][j][e] + bco));
+                    if (p < NP) sd[p * SPF + co] = bf2f(f2bf(gelu(acc[m][j][e] + bco)));
                 }
             }
         }
@@ -842,17 +843,13 @@ __global__ __launch_bounds__(512) void c7_head_flux_kernel(const bf16* __restric
             float jc[3] = {bh[ch], 0.f, 0.f};
 #pragma unroll
             for (int tap = 0; tap < 9; ++tap) {
-                const bf16* sp = sd + ((r + tap / 3 - 1) * DW + (c + tap % 3 - 1)) * SP;
+                const float* sp = sd + ((r + tap / 3 - 1) * DW + (c + tap % 3 - 1)) * SPF;
                 const float* w0 = wh + tap * 64 + 32 * ch;
 #pragma unroll
-                for (int c8 = 0; c8 < 4; ++c8) {
-                    const u32x4 v = *reinterpret_cast<const u32x4*>(sp + 8 * c8);
+                for (int c4 = 0; c4 < 8; ++c4) {
+                    const f32x4 v = *reinterpret_cast<const f32x4*>(sp + 4 * c4);
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        const int ci = 8 * c8 + 2 * e;
-                        jc[tap % 3] = fmaf(lo_f(v[e]), w0[ci], jc[tap % 3]);
-                        jc[tap % 3] = fmaf(hi_f(v[e]), w0[ci + 1], jc[tap % 3]);
-                    }
+                    for (int e = 0; e < 4; ++e) jc[tap % 3] = fmaf(v[e], w0[4 * c4 + e], jc[tap % 3]);
                 }
             }
             sj[2 * i + ch] = kJScale * ((jc[0] + jc[1]) + jc[2]);
