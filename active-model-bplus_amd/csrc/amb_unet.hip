@@ -756,8 +756,14 @@ __global__ __launch_bounds__(512) void c7_head_flux_kernel(const bf16* __restric
     float* sj = reinterpret_cast<float*>(sin);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int q = lane >> 4, l16 = lane & 15;
-    const int tilesx = W / 16, tilesy = H / TYO;
-    const int b = blockIdx.x, tx = b % tilesx, t2 = b / tilesx, ty = t2 % tilesy, n = t2 / tilesy;
+    // XCD-aware order (gfx950 dispatches block b to XCD b % 8): the two 16-column tiles of
+    // each 32-column strip get blocks B and B + half, which share an XCD.  c12_kernel's block
+    // B covers the same strip and rows (TYO = 2 x its 4 half-resolution rows) and has
+    // written its e1 there, and the phi written here is what that c12 block reads next step,
+    // so both stay in one XCD's L2.  half is a multiple of 8 for grids in multiples of 64.
+    const int half = gridDim.x / 2, s32 = blockIdx.x >= half ? 1 : 0, B = blockIdx.x - s32 * half;
+    const int strips = W / 32, tilesy = H / TYO;
+    const int t2 = B / strips, tx = 2 * (B - t2 * strips) + s32, ty = t2 % tilesy, n = t2 / tilesy;
     const int x0 = tx * 16, y0 = ty * TYO;
 
     // stage s1 = up(u1) + e1: every gather of the thread first, then the stores
@@ -1388,7 +1394,7 @@ int forward_infer(amb_surrogate* s, int n, const float* phi, const float* params
     // <SRC, CIN, NB, TY, WM, WN, NB2>: small tiles, so that several blocks share a CU
     // and hide each other's memory latency
     static_assert(kCinReal <= 8, "c1 inference packs the data channels into 8");
-    constexpr int kC12Rows = 4;
+    constexpr int kC12Rows = 4;  // c7_head_flux_kernel<8> pairs its tiles with these (8 full-resolution rows)
     if (H2 % kC12Rows || W2 % 16) return fail(AMB_E_ARG, "surrogate grid does not fit the inference tiles");
     hipLaunchKernelGGL(c12_kernel<kC12Rows>, dim3(n * (H2 / kC12Rows) * (W2 / 16)), dim3(256 * kC12Rows), 0, st, phi,
                        params, s->wf0_8, M + s->b_off[0], s->wf[1], M + s->b_off[1], H, W, s->e1, s->e2);
