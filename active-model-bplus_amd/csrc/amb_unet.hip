@@ -483,18 +483,19 @@ __global__ __launch_bounds__(256 * TY) void c12_kernel(const float* __restrict__
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const int p = 16 * mt + 4 * q + e;
-                    if (p < NP1) {
-                        const bf16 g = f2bf(gelu(acc1[m][j][e] + bco));
-                        s_e1[p * LDE + co] = g;
-                        const int er = p / EC, ec = p - er * EC;
-                        if (er >= 2 && er < 2 * TY + 2 && ec >= 2 && ec < 34)
-                            e1[(((size_t)n * H + fy0 + 1 + er) * W + fx0 + 1 + ec) * 32 + co] = g;
-                    }
+                    if (p < NP1) s_e1[p * LDE + co] = f2bf(gelu(acc1[m][j][e] + bco));
                 }
             }
         }
     }
     __syncthreads();
+
+    // the block's own 2 TY x 32 pixels of c1 to e1, in 16-byte chunks from the LDS tile
+    for (int i = tid; i < 2 * TY * 32 * 4; i += NT) {
+        const int c4 = i & 3, pix = i >> 2, orow = pix >> 5, ocol = pix & 31;
+        *reinterpret_cast<u32x4*>(e1 + (((size_t)n * H + 2 * y0 + orow) * W + 2 * x0 + ocol) * 32 + 8 * c4) =
+            *reinterpret_cast<const u32x4*>(s_e1 + ((orow + 2) * EC + ocol + 2) * LDE + 8 * c4);
+    }
 
     // 2x2 average pooling into c2's operand tile, summed and rounded as load_a<kSrcPool>
     constexpr int ITEMS2 = PR * PW * 4, PER2 = (ITEMS2 + NT - 1) / NT;
