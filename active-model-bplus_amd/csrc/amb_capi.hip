@@ -118,11 +118,13 @@ int validate_params(int64_t ny, int64_t nx, const amb_stencil* st, const amb_mod
 
 int grid_tiles(int64_t n, int t) { return (int)((n + t - 1) / t); }
 
-// Default tile: in order of preference (fastest per site first, measured at 4096^2: the
-// table order, or AMB_F32_NOISELESS_ORDER for the noiseless FP32 kernel), the first shape
-// whose tiles fill the device's resident workgroups either for at least 8 rounds or in
-// whole-or-mostly-full rounds (a last round under half full leaves most CUs idle for a
-// round); fields too small for one round take the shape with the most tiles.
+// Default tile: in order of preference (AMB_F32_NOISY_ORDER, AMB_F32_NOISELESS_ORDER, or
+// the FP64 table order), the first shape whose tiles fill the device's resident workgroups
+// either for at least 8 rounds or in whole-or-mostly-full rounds (a last round under half
+// full leaves most CUs idle for a round).  Fields too small for one round of any shape take
+// the shape whose tile count is nearest two per CU (in ratio; the larger count on a tie):
+// one tile per CU leaves each CU a single workgroup's latency to hide, while many small
+// tiles pay their halo several times over.
 int default_tile(int precision, int64_t ny, int64_t nx, int noise)
 {
     int device = 0, cus = 256;
@@ -131,28 +133,33 @@ int default_tile(int precision, int64_t ny, int64_t nx, int noise)
         (void)hipGetLastError();
         cus = 256;
     }
+    static constexpr int kNoisyF32[] = {AMB_F32_NOISY_ORDER};
     static constexpr int kNoiselessF32[] = {AMB_F32_NOISELESS_ORDER};
 #define AMB_COUNT_ONE(i, tw, th, nt) +1
-    static_assert(sizeof(kNoiselessF32) / sizeof(kNoiselessF32[0]) == 0 AMB_F32_TILES(AMB_COUNT_ONE),
-                  "the noiseless FP32 order must list every FP32 tile shape");
+    static_assert(sizeof(kNoisyF32) / sizeof(kNoisyF32[0]) == 0 AMB_F32_TILES(AMB_COUNT_ONE) &&
+                      sizeof(kNoiselessF32) / sizeof(kNoiselessF32[0]) == 0 AMB_F32_TILES(AMB_COUNT_ONE),
+                  "the FP32 orders must list every FP32 tile shape");
 #undef AMB_COUNT_ONE
     const int n = tile_count(precision);
-    const bool reorder = precision == AMB_F32 && noise == kNoiseNone;
-    int most = 0;
-    int64_t most_tiles = 0;
+    const int* order = precision != AMB_F32 ? nullptr : noise == kNoiseNone ? kNoiselessF32 : kNoisyF32;
+    int best = 0;
+    double best_gap = 0.0;
+    int64_t best_tiles = 0;
     for (int k = 0; k < n; ++k) {
-        const int i = reorder ? kNoiselessF32[k] : k;
+        const int i = order ? order[k] : k;
         const TileShape t = tile_shape(precision, i);
         const int64_t tiles = (int64_t)grid_tiles(nx, t.tw) * grid_tiles(ny, t.th);
         const int64_t slots = (int64_t)cus * tile_occupancy(precision, i, noise);
         const int64_t full = tiles / slots, rest = tiles % slots;
         if (full >= 8 || (full >= 1 && (rest == 0 || 2 * rest >= slots))) return i;
-        if (tiles > most_tiles) {
-            most_tiles = tiles;
-            most = i;
+        const double gap = std::fabs(std::log((double)tiles / (2.0 * cus)));
+        if (k == 0 || gap < best_gap - 1e-9 || (gap < best_gap + 1e-9 && tiles > best_tiles)) {
+            best = i;
+            best_gap = gap;
+            best_tiles = tiles;
         }
     }
-    return most;
+    return best;
 }
 
 // XCD-contiguous tile order: gfx950 dispatches workgroup i to XCD i % 8.

@@ -44,15 +44,18 @@ struct TileShape {
     int tw, th, nt;  // tile columns, tile rows, workgroup size
 };
 
-// Compiled tile configurations (index = autotuner choice): X(index, TW, TH, NT).
-// Index 0 is the fastest measured on MI355X at 4096^2; the default picks the first
-// entry that still gives at least four workgroups per CU.
-#define AMB_F32_TILES(X) X(0, 32, 48, 512) X(1, 32, 24, 256) X(2, 32, 32, 256) X(3, 64, 32, 512) X(4, 64, 16, 256) X(5, 32, 16, 256)
-#define AMB_F64_TILES(X) X(0, 32, 32, 256) X(1, 64, 16, 256) X(2, 32, 16, 256) X(3, 32, 32, 512)
-// The noiseless FP32 kernel holds its 17-tap stencil windows in registers (92 VGPRs instead
-// of 73), so smaller workgroups pack better: its order, fastest first at 4096^2 on MI355X
-// (98.0, 104.4, 106.8, 110.2, 110.4, 121.7 us per step).
-#define AMB_F32_NOISELESS_ORDER 1, 2, 5, 3, 0, 4
+// Compiled tile configurations (index = autotuner choice): X(index, TW, TH, NT).  The
+// default tile follows an order of preference per kernel (default_tile in amb_capi.hip).
+#define AMB_F32_TILES(X) X(0, 32, 48, 512) X(1, 32, 24, 256) X(2, 32, 32, 256) X(3, 64, 32, 512) X(4, 64, 16, 256) X(5, 32, 16, 256) X(6, 16, 16, 128)
+#define AMB_F64_TILES(X) X(0, 32, 32, 256) X(1, 64, 16, 256) X(2, 32, 16, 256) X(3, 32, 32, 512) X(4, 16, 16, 128)
+// FP32 orders of preference, measured on MI355X.  Noisy: 32x48 is fastest at 4096^2
+// (113.8 us per step, 116-139 us for the others) and 64x32 at 1024^2, where a single round
+// of 64x32 workgroups pays least halo (11.0 us, 32x32 11.6 us).  Noiseless: the kernel holds
+// its 17-tap windows in registers (92 VGPRs instead of 73), so 256-thread workgroups pack
+// better; 32x24 is fastest at 4096^2 (98.0 us, 32x48 110.4 us) and 64x32 at 1024^2 (9.3 us).
+// 16x16 serves the smallest fields (256^2: 5.3 us noiseless, 32x16 5.6 us).
+#define AMB_F32_NOISY_ORDER 0, 3, 2, 1, 4, 5, 6
+#define AMB_F32_NOISELESS_ORDER 1, 3, 2, 5, 0, 4, 6
 
 // Tile shapes compiled per precision (index = autotuner choice).
 int tile_count(int precision);

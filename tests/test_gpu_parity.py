@@ -264,14 +264,17 @@ def test_tile_shapes_bit_identical(amb, temperature):
 def test_default_tile_per_noise_mode(amb):
     """Without autotuning the tile comes from the occupancy rule over each kernel's order of
     preference: at 4096^2 FP32 the noisy kernel takes 32x48 tiles of 512 threads, the
-    noiseless kernel (its 17-tap windows need more registers) 32x24 tiles of 256 threads;
-    a 1024^2 field takes 32x32 either way (one full round of resident workgroups)."""
-    shapes = amb.tile_shapes(32)
-    for side, temperature, want in [(4096, 0.05, (32, 48, 512)), (4096, 0.0, (32, 24, 256)),
-                                    (1024, 0.1, (32, 32, 256)), (1024, 0.0, (32, 32, 256))]:
+    noiseless kernel (its 17-tap windows need more registers) 32x24 tiles of 256 threads; a
+    1024^2 field one full round of 64x32 tiles either way; fields too small for a round the
+    shape nearest two tiles per CU (32x16 at 512^2, 16x16 at 256^2, in both precisions)."""
+    for precision, side, temperature, want in [(32, 4096, 0.05, (32, 48, 512)), (32, 4096, 0.0, (32, 24, 256)),
+                                               (32, 1024, 0.1, (64, 32, 512)), (32, 1024, 0.0, (64, 32, 512)),
+                                               (32, 512, 0.05, (32, 16, 256)), (32, 256, 0.0, (16, 16, 128)),
+                                               (64, 256, 0.0, (16, 16, 128))]:
         sim = make(amb, np.zeros((side, side)), (5e-3, 1, 1), (-0.25, 0, 0.25, 1, 0.5, -1, temperature),
-                   precision=32)
-        assert tuple(shapes[sim.options["tile"] - 1]) == want, (side, temperature)
+                   precision=precision)
+        shape = tuple(amb.tile_shapes(precision)[sim.options["tile"] - 1])
+        assert shape == want, (precision, side, temperature, shape)
 
 
 def test_graph_and_direct_launch_agree(amb):
