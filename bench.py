@@ -35,6 +35,9 @@ sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_PROFILE = {"c3": "step_c3_fp32_4096", "c3x8192": "step_c3_fp32_8192"}
+VALU_PROFILE = {"c3": "step_c3_fp32_4096"}
+# Non-packed FP32 vector issue rate of MI355X: 256 CUs x 4 SIMDs x 16 lanes at 2.4 GHz.
+VALU_ISSUE_PEAK = 256 * 64 * 2.4e9
 METRIC = "site-updates/s, % HBM peak; batched vs plain; surrogate speedup at ≤2% S(k) err"
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
@@ -115,21 +118,36 @@ def cpu_baseline(w, phi, budget_s=15.0, budget_1core_s=8.0):
             "value_1core": value1, "host_cpu": f"{host_cpu_model()}, {os.cpu_count()} logical CPUs visible"}
 
 
-def read_traffic(name, tile):
-    """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and
-    calibrated against a plain copy kernel (profiles/<name>_traffic.md), or None when
-    no measurement exists for this workload and tile shape."""
-    path = os.path.join(REPO, "profiles", f"{name}_traffic.md")
+def read_profile_value(path, tile, tile_key, value_key):
+    """The value of `value_key:` in a profile summary under profiles/, or None when the file
+    is missing or its `tile_key:` line names another tile shape than `tile`."""
     if not os.path.exists(path):
         return None
     fields = {}
     for line in open(path):
-        if ":" in line and line.split(":")[0].strip() in ("traffic_tile", "traffic_bytes_per_launch"):
+        if ":" in line and line.split(":")[0].strip() in (tile_key, value_key):
             k, v = line.split(":", 1)
             fields[k.strip()] = v.strip()
-    if fields.get("traffic_tile") != "x".join(str(t) for t in tile):
+    if fields.get(tile_key) != "x".join(str(t) for t in tile) or value_key not in fields:
         return None
-    return int(fields["traffic_bytes_per_launch"])
+    return float(fields[value_key])
+
+
+def read_traffic(name, tile):
+    """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and
+    calibrated against a plain copy kernel (profiles/<name>_traffic.md), or None when
+    no measurement exists for this workload and tile shape."""
+    v = read_profile_value(os.path.join(REPO, "profiles", f"{name}_traffic.md"), tile, "traffic_tile",
+                           "traffic_bytes_per_launch")
+    return None if v is None else int(v)
+
+
+def read_valu(name, tile):
+    """VALU lane-instructions per site measured with rocprofv3 --pmc SQ_INSTS_VALU
+    (profiles/<name>_sq_counters.md), or None when no measurement exists for this
+    workload and tile shape."""
+    return read_profile_value(os.path.join(REPO, "profiles", f"{name}_sq_counters.md"), tile, "valu_tile",
+                              "valu_lane_instructions_per_site")
 
 
 def bench_single(args, key):
@@ -166,6 +184,13 @@ def bench_single(args, key):
         "device_ms_per_step": device_ms / args.steps,
         "mass_fp64": mass, "nonfinite": bad,
     }
+    valu = read_valu(VALU_PROFILE.get(key, ""), sim_tile_shape(w["precision"], tile))
+    if valu is not None:
+        # the bound that holds: vector-ALU issue (the HBM fraction above is the honest one
+        # against 8 TB/s, but the step runs out of VALU issue slots first)
+        issued = valu * sites / launch_s
+        out["roofline"]["valu"] = {"lane_instructions_per_site": valu, "achieved": issued, "peak": VALU_ISSUE_PEAK,
+                                   "unit": "lane-instructions/s", "frac": issued / VALU_ISSUE_PEAK}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, phi)
     return out
