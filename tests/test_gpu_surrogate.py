@@ -38,15 +38,26 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
 
 
-def test_forward_matches_oracle(amb):
-    S = amb.Surrogate(64, 64, max_batch=4, seed=1)
+@pytest.mark.parametrize("h,w", [(64, 64), (64, 128), (128, 64)])
+def test_forward_matches_oracle(amb, h, w):
+    """Also on non-square grids: the inference kernels' block orders (c7 pairs its tiles with
+    c12's strips) depend on the grid's width and height separately.
+
+    Two bars.  Most pixels agree to FP32 rounding (median error < 1e-4 of the RMS current):
+    an indexing mistake would spoil whole tiles.  The relative L2 error is set by bfloat16
+    rounding flips, which the network amplifies: the oracle against itself with its input
+    perturbed by 1e-9 differs by 6.5e-4 (64^2) to 1.5e-3 (64x128) in J, on 17-37% of the
+    pixels, so the bar is 2e-3 at 64^2 and 5e-3 on the larger grids."""
+    S = amb.Surrogate(h, w, max_batch=4, seed=1)
     theta = perturbed(S, 2)
-    phi, params, _ = batch(2, 64, 64, 3)
+    phi, params, _ = batch(2, h, w, 3)
     nxt, J = S.predict(phi, params)
     J_o, nxt_o, _ = U.forward(theta, phi, params, emulate_bf16=True)
     err_j, err_d = rel(J, J_o), rel(nxt - phi, nxt_o - phi)
-    print(f"rel err J {err_j:.2e}, div J {err_d:.2e}")
-    assert err_j < 2e-3
+    med = float(np.median(np.abs(J - J_o)) / np.sqrt(np.mean(J_o ** 2)))
+    print(f"rel err J {err_j:.2e}, div J {err_d:.2e}, median pixel error {med:.1e} of the RMS current")
+    assert med < 1e-4
+    assert err_j < (2e-3 if h * w <= 64 * 64 else 5e-3)
     assert err_d < 1e-2  # the divergence differences neighbouring currents
     # flux form: the sum of the field is unchanged up to FP32 summation
     assert np.all(np.abs(nxt.sum(axis=(1, 2)) - phi.sum(axis=(1, 2))) < 1e-3)
