@@ -30,27 +30,33 @@ __all__ = ["Model", "Stencil", "Integrator", "Ensemble", "Surrogate", "CudaExcep
 
 
 def plot_field(field, fig=None, axes=None, fontsize=8):
-    """Show the evolving field during program execution (reference :14-36)."""
-    import matplotlib.pyplot as plt
-    from matplotlib.colors import Normalize
+    """Live image of a field with a colour bar.
 
-    if fig is not None:
-        ax, im, cbar = axes
-        im.set_data(field)
-        im.set_norm(Normalize(field.min(), vmax=field.max()))
-        return fig, axes
+    The first call (``fig=None``) opens a figure and returns ``(fig, (ax, image,
+    colorbar))``; passing those back redraws the same image with the new field and a
+    colour range stretched to its minimum and maximum.  Same call pattern as the
+    reference's helper (activemodelbplus/integrator/__init__.py:14-36); matplotlib is
+    imported only here.
+    """
+    values = np.asarray(field)
+    lo, hi = float(values.min()), float(values.max())
+    if fig is None:
+        import matplotlib.pyplot as plt
 
-    fig = plt.figure()
-    ax = plt.gca()
-    im = ax.imshow(field)
-    cbar = fig.colorbar(im, ax=ax, fraction=0.05, pad=2e-2, location='right')
-    cbar.set_label(r'$\phi$', fontsize=fontsize)
-    cbar.ax.tick_params(labelsize=fontsize)
-    ax.set_xlabel('$x$', fontsize=fontsize)
-    ax.set_ylabel('$y$', fontsize=fontsize)
-    ax.set_xticks([])
-    ax.set_yticks([])
-    return fig, (ax, im, cbar)
+        fig, ax = plt.subplots()
+        image = ax.imshow(values, vmin=lo, vmax=hi)
+        ax.set_xticks([])
+        ax.set_yticks([])
+        for name, setter in (("$x$", ax.set_xlabel), ("$y$", ax.set_ylabel)):
+            setter(name, fontsize=fontsize)
+        colorbar = fig.colorbar(image, ax=ax, fraction=0.05, pad=0.02)
+        colorbar.set_label(r"$\phi$", fontsize=fontsize)
+        colorbar.ax.tick_params(labelsize=fontsize)
+        return fig, (ax, image, colorbar)
+    image = axes[1]
+    image.set_data(values)
+    image.set_clim(lo, hi)
+    return fig, axes
 
 
 def steps_from_time(dt, tfinal):
@@ -95,65 +101,78 @@ def step_iterator(dt, tfinal, max_updates, min_update_time, t_interrupt=None):
     return step_size
 
 
+class _LiveView:
+    """Progress display of run_for_time: a tqdm bar over the chunks and a redrawn image
+    of the field after each one (matplotlib, tqdm and IPython are needed only here)."""
+
+    def __init__(self, sim):
+        self.sim = sim
+        self.fig, self.axes = plot_field(sim.field)
+
+    def chunks(self, sizes):
+        from tqdm import tqdm
+        return tqdm(sizes)
+
+    def before_chunk(self):
+        sys.stdout.flush()
+        sys.stderr.flush()
+
+    def after_chunk(self):
+        from IPython.display import clear_output, display
+        clear_output(wait=True)
+        plot_field(self.sim.field, self.fig, self.axes)
+        display(self.fig)
+
+    def close(self):
+        import matplotlib.pyplot as plt
+        plt.close(self.fig)
+
+
 class Integrator(BaseIntegrator):
     def run_for_time(self, t, show_progress=False, t_interrupt=None, f_interrupt=None,
                      max_updates=100, min_update_time=10):
-        """Run the simulation for a span of time (reference :105-161).
+        """Advance the simulation by a span of time t (a multiple of the stencil's dt).
 
-        Args:
-            t: time to run for; a multiple of the stencil's dt.
-            show_progress: show a live plot and a progress bar (needs matplotlib,
-                tqdm and IPython).
-            t_interrupt: times (multiples of dt, within [0, t]) at which
-                ``f_interrupt(self)`` is called.
+        The span is cut into chunks by ``step_iterator`` (at most ``max_updates`` of them,
+        each at least ``min_update_time`` long, plus a cut at every interrupt time); each
+        chunk is one ``run(n)``.  ``f_interrupt(self)`` is called whenever the time elapsed
+        since the start matches one of ``t_interrupt``: before a chunk, and once more
+        after the last one.  Behaviour of the reference's run_for_time
+        (activemodelbplus/integrator/__init__.py:105-161).
+
         Raises:
-            ValueError: if t is not a multiple of the timestep, or an interrupt
-                lies outside [0, t].
+            ValueError: if t (or an interrupt time) is not a multiple of dt, or an
+                interrupt lies outside [0, t].
         """
-        nsteps = steps_from_time(self.stencil.dt, t)
-
+        dt = self.stencil.dt
+        total = steps_from_time(dt, t)
+        samples = None
         if t_interrupt is not None:
             assert f_interrupt is not None
-            if np.any(np.asarray(t_interrupt) < 0) or np.any(np.asarray(t_interrupt) > t):
+            samples = np.asarray(t_interrupt, dtype=float)
+            if (samples < 0).any() or (samples > t).any():
                 raise ValueError('interrupt outside sim time window!')
+        sizes = step_iterator(dt, t, max_updates, min_update_time, t_interrupt=t_interrupt)
 
-        initial_timestep = self.timestep
-        initial_time = self.time
-        if show_progress:
-            fig, axes = plot_field(self.field)
+        start_step, start_time = self.timestep, self.time
 
-        steps = step_iterator(self.stencil.dt, t, max_updates, min_update_time,
-                              t_interrupt=t_interrupt)
-        if show_progress:
-            from tqdm import tqdm
-            steps = tqdm(steps)
+        def sample_if_due():
+            if samples is not None and np.isclose(self.time - start_time, samples).any():
+                f_interrupt(self)
 
-        for n in steps:
-            if show_progress:
-                for out in [sys.stdout, sys.stderr]:
-                    out.flush()
-
-            if t_interrupt is not None:
-                if np.any(np.isclose(self.time - initial_time, t_interrupt)):
-                    f_interrupt(self)
-
+        view = _LiveView(self) if show_progress else None
+        for n in (view.chunks(sizes) if view else sizes):
+            if view:
+                view.before_chunk()
+            sample_if_due()
             assert n > 0
             self.run(int(n))
-
-            if show_progress:
-                from IPython.display import clear_output, display
-                clear_output(wait=True)
-                plot_field(self.field, fig, axes)
-                display(fig)
-
-        if show_progress:
-            import matplotlib.pyplot as plt
-            plt.close()
-        assert (self.timestep - initial_timestep) == nsteps
-
-        if t_interrupt is not None:
-            if np.any(np.isclose(self.time - initial_time, t_interrupt)):
-                f_interrupt(self)
+            if view:
+                view.after_chunk()
+        if view:
+            view.close()
+        assert self.timestep - start_step == total
+        sample_if_due()
 
     # -- exact resume (no reference counterpart; the reference reseeds on every
     #    construction, src/integrator.cu:320-323) --------------------------------

@@ -35,16 +35,23 @@ struct BatchDesc {
 template <typename T, int TW, int TH, int NT>
 __global__ __launch_bounds__(NT) void batch_step_kernel(const BatchDesc<T>* __restrict__ desc,
                                                               const int2* __restrict__ tiles, const T* in, T* out,
-                                                              const long long* step_base, int step_off)
+                                                              const long long* step_base, int step_off, double* part)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int2 t = tiles[blockIdx.x];
-    if (t.x < 0) return;  // padding entry of the XCD-ordered list
+    double* const red = part ? part + 2 * (size_t)blockIdx.x : nullptr;
+    if (t.x < 0) {  // padding entry of the XCD-ordered list
+        if (red && threadIdx.x == 0) {
+            red[0] = 0.0;
+            red[1] = 0.0;
+        }
+        return;
+    }
     const BatchDesc<T>& d = desc[t.x];
     const int ty = t.y / d.ntx, tx = t.y - ty * d.ntx;
     const uint64_t step = (uint64_t)(*step_base) + (uint64_t)step_off;
     step_tile<T, TW, TH, NT, kNoiseRuntime>(reinterpret_cast<T*>(smem_raw), in + d.off, out + d.off, d.ny, d.nx,
-                                            d.nx4, tx * TW, ty * TH, d.k, d.key0, d.key1, step, nullptr);
+                                            d.nx4, tx * TW, ty * TH, d.k, d.key0, d.key1, step, nullptr, red);
 }
 
 template <typename T, int TW, int TH, int NT>
@@ -70,22 +77,25 @@ static hipError_t prepare_batch_kernels()
 
 template <typename T, int TW, int TH, int NT>
 static hipError_t launch_batch_one(const void* desc, const int2* tiles, int ntiles, const void* in, void* out,
-                                   const long long* step, int step_off, hipStream_t s)
+                                   const long long* step, int step_off, double* part, hipStream_t s)
 {
     using L = Lds<T, TW, TH, NT>;
     hipLaunchKernelGGL((batch_step_kernel<T, TW, TH, NT>), dim3(ntiles), dim3(NT), L::bytes, s,
                        static_cast<const BatchDesc<T>*>(desc), tiles, static_cast<const T*>(in), static_cast<T*>(out),
-                       step, step_off);
+                       step, step_off, part);
     return hipGetLastError();
 }
 
 #define AMB_BCASE_F32(i, tw, th, nt) \
-    case i: return launch_batch_one<float, tw, th, nt>(desc, tiles, ntiles, in, out, step, step_off, s);
+    case i: return launch_batch_one<float, tw, th, nt>(desc, tiles, ntiles, in, out, step, step_off, part, s);
 #define AMB_BCASE_F64(i, tw, th, nt) \
-    case i: return launch_batch_one<double, tw, th, nt>(desc, tiles, ntiles, in, out, step, step_off, s);
+    case i: return launch_batch_one<double, tw, th, nt>(desc, tiles, ntiles, in, out, step, step_off, part, s);
 
+// part (nullable): the per-tile partials of the fused finite check, one entry per entry of
+// `tiles` (padding entries write zeros).
 static hipError_t launch_batch(int precision, int tile, const void* desc, const int2* tiles, int ntiles,
-                               const void* in, void* out, const long long* step, int step_off, hipStream_t s)
+                               const void* in, void* out, const long long* step, int step_off, double* part,
+                               hipStream_t s)
 {
     if (precision == AMB_F32) {
         switch (tile) {
@@ -112,6 +122,7 @@ struct Group {
     std::vector<int> members;
     int2* d_tiles = nullptr;
     int ntiles = 0;
+    int64_t part_base = 0;  // first entry of this group's tiles in the partials buffer
 };
 
 }  // namespace
@@ -131,6 +142,11 @@ struct amb_batch {
     std::vector<int> tile_of;  // tile index per realisation
     std::vector<Group> groups;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
+    double* d_tile_part = nullptr;  // (sum, non-finite count) per tile of the last step of a run
+    int64_t ntiles_total = 0;
+    int64_t tile_part_cap = 0;      // entries allocated in d_tile_part
+    double last_sum = 0.0;          // total mass and non-finite count after the last run
+    int64_t last_bad = 0;
     double* d_part = nullptr;
     unsigned long long* d_bad_part = nullptr;
     double* d_sum = nullptr;
@@ -164,7 +180,7 @@ void release(amb_batch* b)
     drop_graphs(b);
     free_groups(b);
     for (void* p : {b->buf[0], b->buf[1], (void*)b->d_step, (void*)b->d_step_scratch, b->d_desc, (void*)b->d_part,
-                    (void*)b->d_bad_part, (void*)b->d_sum, (void*)b->d_bad})
+                    (void*)b->d_bad_part, (void*)b->d_sum, (void*)b->d_bad, (void*)b->d_tile_part})
         if (p) (void)hipFree(p);
     if (b->h_sum) (void)hipHostFree(b->h_sum);
     if (b->h_bad) (void)hipHostFree(b->h_bad);
@@ -199,12 +215,15 @@ int rebuild_groups(amb_batch* b)
     free_groups(b);
     std::map<int, std::vector<int>> by_tile;
     for (size_t r = 0; r < b->desc.size(); ++r) by_tile[b->tile_of[r]].push_back((int)r);
+    b->ntiles_total = 0;
     for (auto& kv : by_tile) {
         Group g;
         g.tile = kv.first;
         g.members = kv.second;
         const std::vector<int2> list = tile_list(b, g.members, g.tile);
         g.ntiles = (int)list.size();
+        g.part_base = b->ntiles_total;
+        b->ntiles_total += g.ntiles;
         hipError_t e = hipMalloc(&g.d_tiles, list.size() * sizeof(int2));
         if (e == hipSuccess) e = hipMemcpy(g.d_tiles, list.data(), list.size() * sizeof(int2), hipMemcpyHostToDevice);
         if (e != hipSuccess) {
@@ -212,6 +231,14 @@ int rebuild_groups(amb_batch* b)
             return fail_hip(e, "tile list upload");
         }
         b->groups.push_back(std::move(g));
+    }
+    if (b->ntiles_total > b->tile_part_cap) {
+        if (b->d_tile_part) (void)hipFree(b->d_tile_part);
+        b->d_tile_part = nullptr;
+        b->tile_part_cap = 0;
+        const hipError_t e = hipMalloc(&b->d_tile_part, (size_t)b->ntiles_total * 2 * sizeof(double));
+        if (e != hipSuccess) return fail_hip(e, "hipMalloc (tile partials)");
+        b->tile_part_cap = b->ntiles_total;
     }
     return AMB_OK;
 }
@@ -250,11 +277,14 @@ int upload_descs(amb_batch* b)
     return e == hipSuccess ? AMB_OK : fail_hip(e, "descriptor upload");
 }
 
-hipError_t launch_groups(amb_batch* b, int parity_in, const long long* step, int step_off)
+// One step of every group; with `reduce` each tile also writes its partial of the fused
+// finite check (the last step of a run).
+hipError_t launch_groups(amb_batch* b, int parity_in, const long long* step, int step_off, bool reduce)
 {
     for (const Group& g : b->groups) {
         const hipError_t e = launch_batch(b->precision, g.tile, b->d_desc, g.d_tiles, g.ntiles, b->buf[parity_in & 1],
-                                          b->buf[(parity_in + 1) & 1], step, step_off, b->stream);
+                                          b->buf[(parity_in + 1) & 1], step, step_off,
+                                          reduce ? b->d_tile_part + 2 * g.part_base : nullptr, b->stream);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -265,7 +295,8 @@ int build_graph(amb_batch* b, int parity)
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return fail_hip(e, "hipStreamBeginCapture");
-    for (int k = 0; k < kBatchGraphSteps && e == hipSuccess; ++k) e = launch_groups(b, parity + k, b->d_step, k);
+    for (int k = 0; k < kBatchGraphSteps && e == hipSuccess; ++k)
+        e = launch_groups(b, parity + k, b->d_step, k, k == kBatchGraphSteps - 1);
     if (e == hipSuccess) e = launch_advance(b->d_step, kBatchGraphSteps, b->stream);
     const hipError_t e2 = hipStreamEndCapture(b->stream, &g);
     if (e != hipSuccess) {
@@ -297,7 +328,8 @@ int enqueue(amb_batch* b, int64_t nsteps)
         left -= reps * kBatchGraphSteps;
     }
     for (int k = 0; k < left; ++k)
-        if ((e = launch_groups(b, b->cur + k, b->d_step, k)) != hipSuccess) return fail_hip(e, "batched launch");
+        if ((e = launch_groups(b, b->cur + k, b->d_step, k, k == left - 1)) != hipSuccess)
+            return fail_hip(e, "batched launch");
     if (left > 0) {
         if ((e = launch_advance(b->d_step, left, b->stream)) != hipSuccess) return fail_hip(e, "advance launch");
         b->cur = (int)((b->cur + left) & 1);
@@ -312,6 +344,25 @@ int reduce_range(amb_batch* b, int64_t off, int64_t n, double* sum, int64_t* bad
                                            b->d_bad_part, b->d_sum, b->d_bad, b->stream)
                        : launch_reduce_f64(static_cast<const double*>(b->buf[b->cur]) + off, n, b->d_part,
                                            b->d_bad_part, b->d_sum, b->d_bad, b->stream);
+    if (e != hipSuccess) return fail_hip(e, "reduction launch");
+    if ((e = hipMemcpyAsync(b->h_sum, b->d_sum, sizeof(double), hipMemcpyDeviceToHost, b->stream)) != hipSuccess)
+        return fail_hip(e, "hipMemcpyAsync");
+    if ((e = hipMemcpyAsync(b->h_bad, b->d_bad, sizeof(unsigned long long), hipMemcpyDeviceToHost, b->stream)) !=
+        hipSuccess)
+        return fail_hip(e, "hipMemcpyAsync");
+    if ((e = hipStreamSynchronize(b->stream)) != hipSuccess) return fail_hip(e, "hipStreamSynchronize");
+    *sum = *b->h_sum;
+    *bad = (int64_t)*b->h_bad;
+    return AMB_OK;
+}
+
+// Mass and non-finite count after a run of nsteps: from the per-tile partials that its
+// last step wrote (one small fixed-order reduction), or, for nsteps = 0, by reading
+// every field.
+int finish_run(amb_batch* b, int64_t nsteps, double* sum, int64_t* bad)
+{
+    if (nsteps == 0) return reduce_range(b, 0, b->total, sum, bad);
+    hipError_t e = launch_reduce_tiles(b->d_tile_part, b->ntiles_total, b->d_sum, b->d_bad, b->stream);
     if (e != hipSuccess) return fail_hip(e, "reduction launch");
     if ((e = hipMemcpyAsync(b->h_sum, b->d_sum, sizeof(double), hipMemcpyDeviceToHost, b->stream)) != hipSuccess)
         return fail_hip(e, "hipMemcpyAsync");
@@ -425,7 +476,9 @@ int amb_batch_run(amb_batch* b, int64_t nsteps)
     if (rc != AMB_OK) return rc;
     double s;
     int64_t bad;
-    if ((rc = reduce_range(b, 0, b->total, &s, &bad)) != AMB_OK) return rc;
+    if ((rc = finish_run(b, nsteps, &s, &bad)) != AMB_OK) return rc;
+    b->last_sum = s;
+    b->last_bad = bad;
     if (bad != 0) return fail(AMB_E_NONFINITE, "an unknown numerical error occurred during simulation");
     return AMB_OK;
 }
@@ -446,7 +499,9 @@ int amb_batch_run_timed(amb_batch* b, int64_t nsteps, double* device_ms)
     *device_ms = ms;
     double s;
     int64_t bad;
-    if ((rc = reduce_range(b, 0, b->total, &s, &bad)) != AMB_OK) return rc;
+    if ((rc = finish_run(b, nsteps, &s, &bad)) != AMB_OK) return rc;
+    b->last_sum = s;
+    b->last_bad = bad;
     if (bad != 0) return fail(AMB_E_NONFINITE, "an unknown numerical error occurred during simulation");
     return AMB_OK;
 }
@@ -482,6 +537,15 @@ int amb_batch_mass(amb_batch* b, int64_t index, double* sum, int64_t* nonfinite)
     if (rc != AMB_OK) return rc;
     if (sum) *sum = s;
     if (nonfinite) *nonfinite = bad;
+    return AMB_OK;
+}
+
+int amb_batch_last_mass(amb_batch* b, double* sum, int64_t* nonfinite)
+{
+    clear_error();
+    if (!b) return fail(AMB_E_ARG, "batch must not be null");
+    if (sum) *sum = b->last_sum;
+    if (nonfinite) *nonfinite = b->last_bad;
     return AMB_OK;
 }
 
@@ -551,7 +615,7 @@ int amb_batch_autotune(amb_batch* b, int reps, int* chosen)
                 if (k == 1) e = hipEventRecord(b->ev0, b->stream);
                 if (e == hipSuccess)
                     e = launch_batch(b->precision, t, b->d_desc, d_tiles, (int)list.size(), b->buf[(b->cur + k) & 1],
-                                     b->buf[(b->cur + k + 1) & 1], b->d_step_scratch, 0, b->stream);
+                                     b->buf[(b->cur + k + 1) & 1], b->d_step_scratch, 0, nullptr, b->stream);
             }
             if (e == hipSuccess) e = hipEventRecord(b->ev1, b->stream);
             if (e == hipSuccess) e = hipEventSynchronize(b->ev1);

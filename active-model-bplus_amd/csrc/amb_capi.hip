@@ -8,6 +8,7 @@
 // state and launches three kernels per step from a host loop.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -184,8 +185,11 @@ struct amb_handle {
     int64_t timestep = 0;
     double* d_part = nullptr;
     unsigned long long* d_bad_part = nullptr;
+    double* d_tile_part = nullptr;         // (sum, non-finite count) per tile, written by a run's last step
     double* d_sum = nullptr;
     unsigned long long* d_bad = nullptr;
+    double last_sum = 0.0;                 // mass and non-finite count found by the last amb_run
+    int64_t last_bad = 0;
     double* h_sum = nullptr;               // pinned
     unsigned long long* h_bad = nullptr;   // pinned
     double* d_obs = nullptr;               // 10 n doubles, allocated by the first getter
@@ -196,6 +200,7 @@ struct amb_handle {
     int graph_tile[2] = {-1, -1};
     int graph_noise[2] = {-1, -1};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    SkCache* sk = nullptr;                 // structure-factor plan and buffers (lazy)
 };
 
 namespace {
@@ -203,7 +208,7 @@ namespace {
 constexpr int kGraphSteps = 32;  // even: a graph returns the field to the buffer it started in
 
 template <typename T>
-StepArgs<T> step_args(const amb_handle* h, int parity_in, int step_off, int noise_mode)
+StepArgs<T> step_args(const amb_handle* h, int parity_in, int step_off, int noise_mode, bool reduce = false)
 {
     StepArgs<T> a{};
     const TileShape ts = tile_shape(h->precision, h->tile);
@@ -223,14 +228,25 @@ StepArgs<T> step_args(const amb_handle* h, int parity_in, int step_off, int nois
     a.noise = nullptr;
     if (noise_mode == kNoiseBuffer)
         a.noise = h->d_noise + (size_t)h->noise_used * 2 * (size_t)(h->ny * h->nx);
+    a.part = reduce ? h->d_tile_part : nullptr;
     return a;
 }
 
-hipError_t launch_step(const amb_handle* h, int parity_in, int step_off, int noise_mode)
+// One step; with `reduce` (the last step of a run) every tile also writes its partial
+// of the fused finite check and FP64 mass.
+hipError_t launch_step(const amb_handle* h, int parity_in, int step_off, int noise_mode, bool reduce = false)
 {
     if (h->precision == AMB_F32)
-        return launch_step_f32(step_args<float>(h, parity_in, step_off, noise_mode), h->tile, noise_mode, h->stream);
-    return launch_step_f64(step_args<double>(h, parity_in, step_off, noise_mode), h->tile, noise_mode, h->stream);
+        return launch_step_f32(step_args<float>(h, parity_in, step_off, noise_mode, reduce), h->tile, noise_mode,
+                               h->stream);
+    return launch_step_f64(step_args<double>(h, parity_in, step_off, noise_mode, reduce), h->tile, noise_mode,
+                           h->stream);
+}
+
+int64_t tiles_of(const amb_handle* h, int tile)
+{
+    const TileShape ts = tile_shape(h->precision, tile);
+    return (int64_t)grid_tiles(h->nx, ts.tw) * grid_tiles(h->ny, ts.th);
 }
 
 int noise_mode_of(const amb_handle* h)
@@ -254,7 +270,8 @@ int build_graph(amb_handle* h, int parity, int noise_mode)
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return fail_hip(e, "hipStreamBeginCapture");
-    for (int k = 0; k < kGraphSteps && e == hipSuccess; ++k) e = launch_step(h, parity + k, k, noise_mode);
+    for (int k = 0; k < kGraphSteps && e == hipSuccess; ++k)
+        e = launch_step(h, parity + k, k, noise_mode, k == kGraphSteps - 1);
     if (e == hipSuccess) e = launch_advance(h->d_step, kGraphSteps, h->stream);
     const hipError_t e2 = hipStreamEndCapture(h->stream, &g);
     if (e != hipSuccess) {
@@ -281,7 +298,7 @@ int enqueue_steps(amb_handle* h, int64_t nsteps)
     while (left > 0) {
         const int mode = noise_mode_of(h);
         if (mode == kNoiseBuffer) {
-            if ((e = launch_step(h, h->cur, 0, mode)) != hipSuccess) return fail_hip(e, "step launch");
+            if ((e = launch_step(h, h->cur, 0, mode, true)) != hipSuccess) return fail_hip(e, "step launch");
             if ((e = launch_advance(h->d_step, 1, h->stream)) != hipSuccess) return fail_hip(e, "advance launch");
             h->cur ^= 1;
             ++h->noise_used;
@@ -301,7 +318,7 @@ int enqueue_steps(amb_handle* h, int64_t nsteps)
             continue;
         }
         for (int k = 0; k < left; ++k)
-            if ((e = launch_step(h, h->cur + k, k, mode)) != hipSuccess) return fail_hip(e, "step launch");
+            if ((e = launch_step(h, h->cur + k, k, mode, k == left - 1)) != hipSuccess) return fail_hip(e, "step launch");
         if ((e = launch_advance(h->d_step, left, h->stream)) != hipSuccess) return fail_hip(e, "advance launch");
         h->cur = (int)((h->cur + left) & 1);
         left = 0;
@@ -329,13 +346,35 @@ int reduce_now(amb_handle* h, double* sum, int64_t* bad)
     return AMB_OK;
 }
 
+// Mass and non-finite count after a run of nsteps: the last step of the run wrote one
+// partial per tile, summed here by one small fixed-order reduction; for nsteps = 0 the
+// field is read in full.
+int finish_run(amb_handle* h, int64_t nsteps, double* sum, int64_t* bad)
+{
+    if (nsteps == 0) return reduce_now(h, sum, bad);
+    hipError_t e = launch_reduce_tiles(h->d_tile_part, tiles_of(h, h->tile), h->d_sum, h->d_bad, h->stream);
+    if (e != hipSuccess) return fail_hip(e, "reduction launch");
+    if ((e = hipMemcpyAsync(h->h_sum, h->d_sum, sizeof(double), hipMemcpyDeviceToHost, h->stream)) != hipSuccess)
+        return fail_hip(e, "hipMemcpyAsync");
+    if ((e = hipMemcpyAsync(h->h_bad, h->d_bad, sizeof(unsigned long long), hipMemcpyDeviceToHost, h->stream)) !=
+        hipSuccess)
+        return fail_hip(e, "hipMemcpyAsync");
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return fail_hip(e, "hipStreamSynchronize");
+    *sum = *h->h_sum;
+    *bad = (int64_t)*h->h_bad;
+    h->last_sum = *sum;
+    h->last_bad = *bad;
+    return AMB_OK;
+}
+
 void release(amb_handle* h)
 {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     drop_graphs(h);
+    sk_cache_free(h->sk);
     for (void* p : {h->buf[0], h->buf[1], (void*)h->d_step, (void*)h->d_part, (void*)h->d_bad_part,
-                    (void*)h->d_sum, (void*)h->d_bad, (void*)h->d_obs, (void*)h->d_noise})
+                    (void*)h->d_tile_part, (void*)h->d_sum, (void*)h->d_bad, (void*)h->d_obs, (void*)h->d_noise})
         if (p) (void)hipFree(p);
     if (h->h_sum) (void)hipHostFree(h->h_sum);
     if (h->h_bad) (void)hipHostFree(h->h_bad);
@@ -359,22 +398,24 @@ int upload_field(amb_handle* h, const double* field, int which)
     return e == hipSuccess ? AMB_OK : fail_hip(e, "hipMemcpy (field upload)");
 }
 
-int compute_observables(amb_handle* h)
+// Computes observable `which` into d_obs + 2n (workspace d_obs[0, 2n)); allocated on
+// the first getter call.
+int compute_observable(amb_handle* h, int which)
 {
     const size_t n = (size_t)(h->ny * h->nx);
     hipError_t e;
     if (!h->d_obs) {
-        if ((e = hipMalloc(&h->d_obs, 10 * n * sizeof(double))) != hipSuccess) return fail_hip(e, "hipMalloc");
+        if ((e = hipMalloc(&h->d_obs, 4 * n * sizeof(double))) != hipSuccess) return fail_hip(e, "hipMalloc");
     }
     const int mode = noise_mode_of(h);
     if (h->precision == AMB_F32) {
         StepArgs<float> a = step_args<float>(h, h->cur, 0, mode);
         if (mode != kNoiseBuffer) a.noise = nullptr;
-        e = launch_observables_f32(a, h->d_obs, h->stream);
+        e = launch_observable_f32(a, which, h->d_obs, h->d_obs + 2 * n, h->stream);
     } else {
         StepArgs<double> a = step_args<double>(h, h->cur, 0, mode);
         if (mode != kNoiseBuffer) a.noise = nullptr;
-        e = launch_observables_f64(a, h->d_obs, h->stream);
+        e = launch_observable_f64(a, which, h->d_obs, h->d_obs + 2 * n, h->stream);
     }
     if (e != hipSuccess) return fail_hip(e, "observable launch");
     return AMB_OK;
@@ -423,6 +464,8 @@ int handle_set_timestep(amb_handle* h, int64_t timestep)
 }
 
 int64_t handle_timestep(const amb_handle* h) { return h->timestep; }
+
+SkCache** handle_sk_cache(amb_handle* h) { return &h->sk; }
 
 }  // namespace amb
 
@@ -516,6 +559,11 @@ int amb_create(const double* field, int64_t ny, int64_t nx, const amb_stencil* s
     AMB_CHECK(hipMalloc(&h->d_step, sizeof(long long)));
     AMB_CHECK(hipMalloc(&h->d_part, kReduceBlocks * sizeof(double)));
     AMB_CHECK(hipMalloc(&h->d_bad_part, kReduceBlocks * sizeof(unsigned long long)));
+    {
+        int64_t most = 1;  // partials for the tile shape with the most tiles (the tile may change)
+        for (int t = 0; t < tile_count(h->precision); ++t) most = std::max(most, tiles_of(h, t));
+        AMB_CHECK(hipMalloc(&h->d_tile_part, (size_t)most * 2 * sizeof(double)));
+    }
     AMB_CHECK(hipMalloc(&h->d_sum, sizeof(double)));
     AMB_CHECK(hipMalloc(&h->d_bad, sizeof(unsigned long long)));
     AMB_CHECK(hipHostMalloc(&h->h_sum, sizeof(double), hipHostMallocDefault));
@@ -551,7 +599,7 @@ int amb_run(amb_handle* h, int64_t nsteps)
     h->timestep += nsteps;
     double sum;
     int64_t bad;
-    if ((rc = reduce_now(h, &sum, &bad)) != AMB_OK) return rc;
+    if ((rc = finish_run(h, nsteps, &sum, &bad)) != AMB_OK) return rc;
     if (bad != 0) return fail(AMB_E_NONFINITE, "an unknown numerical error occurred during simulation");
     return AMB_OK;
 }
@@ -573,7 +621,7 @@ int amb_run_timed(amb_handle* h, int64_t nsteps, double* device_ms)
     h->timestep += nsteps;
     double sum;
     int64_t bad;
-    if ((rc = reduce_now(h, &sum, &bad)) != AMB_OK) return rc;
+    if ((rc = finish_run(h, nsteps, &sum, &bad)) != AMB_OK) return rc;
     if (bad != 0) return fail(AMB_E_NONFINITE, "an unknown numerical error occurred during simulation");
     return AMB_OK;
 }
@@ -600,39 +648,13 @@ int amb_get_observable(amb_handle* h, int which, double* out)
     clear_error();
     if (!h || !out) return fail(AMB_E_ARG, "handle and out must not be null");
     if (which < AMB_OBS_MU || which > AMB_OBS_J_RANDOM) return fail(AMB_E_ARG, "unknown observable");
-    int rc = compute_observables(h);
+    const int rc = compute_observable(h, which);
     if (rc != AMB_OK) return rc;
     const size_t n = (size_t)(h->ny * h->nx);
-    std::vector<double> all(10 * n);
-    hipError_t e = hipStreamSynchronize(h->stream);
-    if (e == hipSuccess) e = hipMemcpy(all.data(), h->d_obs, 10 * n * sizeof(double), hipMemcpyDeviceToHost);
-    if (e != hipSuccess) return fail_hip(e, "hipMemcpy (observables)");
-    const double* mup = all.data();
-    const double* mua = mup + n;
-    const double* jp = mup + 2 * n;
-    const double* jl = mup + 4 * n;
-    const double* jz = mup + 6 * n;
-    const double* jn = mup + 8 * n;
-    switch (which) {
-    case AMB_OBS_MU:
-        for (size_t i = 0; i < n; ++i) out[i] = mup[i] + mua[i];
-        break;
-    case AMB_OBS_MU_PASSIVE: std::memcpy(out, mup, n * sizeof(double)); break;
-    case AMB_OBS_MU_ACTIVE: std::memcpy(out, mua, n * sizeof(double)); break;
-    case AMB_OBS_J_PASSIVE: std::memcpy(out, jp, 2 * n * sizeof(double)); break;
-    case AMB_OBS_J_LAMBDA: std::memcpy(out, jl, 2 * n * sizeof(double)); break;
-    case AMB_OBS_J_CIRCULATING: std::memcpy(out, jz, 2 * n * sizeof(double)); break;
-    case AMB_OBS_J_RANDOM: std::memcpy(out, jn, 2 * n * sizeof(double)); break;
-    case AMB_OBS_J_ACTIVE:
-        for (size_t i = 0; i < 2 * n; ++i) out[i] = jz[i] + jl[i];
-        break;
-    case AMB_OBS_J_DETERMINISTIC:
-        for (size_t i = 0; i < 2 * n; ++i) out[i] = (jz[i] + jl[i]) + jp[i];
-        break;
-    case AMB_OBS_J:
-        for (size_t i = 0; i < 2 * n; ++i) out[i] = ((jz[i] + jl[i]) + jp[i]) + jn[i];
-        break;
-    }
+    const size_t count = which <= AMB_OBS_MU_ACTIVE ? n : 2 * n;
+    hipError_t e = hipMemcpyAsync(out, h->d_obs + 2 * n, count * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return fail_hip(e, "hipMemcpy (observable)");
     return AMB_OK;
 }
 
@@ -678,6 +700,15 @@ int amb_mass(amb_handle* h, double* sum, int64_t* nonfinite)
     if (rc != AMB_OK) return rc;
     if (sum) *sum = s;
     if (nonfinite) *nonfinite = b;
+    return AMB_OK;
+}
+
+int amb_last_mass(amb_handle* h, double* sum, int64_t* nonfinite)
+{
+    clear_error();
+    if (!h) return fail(AMB_E_ARG, "handle must not be null");
+    if (sum) *sum = h->last_sum;
+    if (nonfinite) *nonfinite = h->last_bad;
     return AMB_OK;
 }
 

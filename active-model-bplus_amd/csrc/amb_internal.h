@@ -41,4 +41,10 @@ int handle_enqueue(amb_handle* h, int64_t steps);
 int handle_set_timestep(amb_handle* h, int64_t timestep);
 int64_t handle_timestep(const amb_handle* h);
 
+// Per-handle state of the structure factor (FFT plan and buffers), created by
+// amb_structure_factor on first use and freed with the handle.
+struct SkCache;
+SkCache** handle_sk_cache(amb_handle* h);
+void sk_cache_free(SkCache* c);
+
 }  // namespace amb

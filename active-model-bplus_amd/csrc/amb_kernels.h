@@ -35,6 +35,7 @@ struct StepArgs {
     const long long* step_base;  // device step counter; noise uses *step_base + step_off
     int step_off;
     const double* noise;         // explicit normals [2][ny][nx] of this step (noise mode 2)
+    double* part;                // nullable: per-tile (sum, non-finite count) of the new field
 };
 
 // kNoiseRuntime: Philox when the coefficient amp is non-zero (batched realisations).
@@ -70,10 +71,15 @@ int tile_occupancy(int precision, int tile, int noise);  // resident workgroups 
 // Adds `n` to the device step counter (a one-lane kernel, graph-capturable).
 hipError_t launch_advance(long long* step, long long n, hipStream_t s);
 
-// Observables of the current step into double buffers (each n = ny*nx):
-// mup[n], mua[n], jp[2n], jl[2n], jz[2n], jn[2n].
-hipError_t launch_observables_f32(const StepArgs<float>& a, double* obs, hipStream_t s);
-hipError_t launch_observables_f64(const StepArgs<double>& a, double* obs, hipStream_t s);
+// One observable (AMB_OBS_*) of the current step into `out` (n = ny*nx doubles for the
+// chemical potentials, 2n for currents); `tmp` is 2n doubles of workspace.
+hipError_t launch_observable_f32(const StepArgs<float>& a, int which, double* tmp, double* out, hipStream_t s);
+hipError_t launch_observable_f64(const StepArgs<double>& a, int which, double* tmp, double* out, hipStream_t s);
+
+// Final reduction of the per-tile partials (sum, non-finite count) that the last step of
+// a run writes: one block, fixed order.
+hipError_t launch_reduce_tiles(const double* part, int64_t ntiles, double* sum, unsigned long long* bad,
+                               hipStream_t s);
 
 // FP64 sum and non-finite count of a field: deterministic two-pass reduction.
 constexpr int kReduceBlocks = 512;

@@ -36,7 +36,7 @@ __global__ __launch_bounds__(NT) void step_kernel(const StepArgs<T> p)
     const int ty = t / p.ntx, tx = t - ty * p.ntx;
     const uint64_t step = NOISE == kNoisePhilox ? (uint64_t)(*p.step_base) + (uint64_t)p.step_off : 0;
     step_tile<T, TW, TH, NT, NOISE>(reinterpret_cast<T*>(smem_raw), p.in, p.out, p.ny, p.nx, p.nx4, tx * TW, ty * TH,
-                                    p.k, p.key0, p.key1, step, p.noise);
+                                    p.k, p.key0, p.key1, step, p.noise, p.part ? p.part + 2 * (size_t)t : nullptr);
 }
 
 // ---- tile table ---------------------------------------------------------------
@@ -181,14 +181,12 @@ __device__ __forceinline__ T ld(const T* f, int ny, int nx, int r, int c)
     return f[(size_t)wrap_index(r, ny) * nx + wrap_index(c, nx)];
 }
 
+// Local terms at one site from its 3x3 neighbourhood: mu_p, mu_a and the circulating
+// current (y, x components), in the arithmetic type of the field.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void obs_local_kernel(const StepArgs<T> p, double* obs)
+__device__ __forceinline__ void local_terms(const StepArgs<T>& p, int r, int c, T& mup, T& mua, T& jzy, T& jzx)
 {
     const int ny = p.ny, nx = p.nx;
-    const size_t n = (size_t)ny * nx;
-    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
-    if (i >= n) return;
-    const int r = (int)(i / nx), c = (int)(i - (size_t)r * nx);
     const StepCoef<T>& K = p.k;
     const T* f = p.in;
     const T pmm = ld(f, ny, nx, r - 1, c - 1), pm0 = ld(f, ny, nx, r - 1, c), pmp = ld(f, ny, nx, r - 1, c + 1);
@@ -197,30 +195,56 @@ __global__ __launch_bounds__(kThreads) void obs_local_kernel(const StepArgs<T> p
     const T gx = K.gxe * ((pmp - pmm) + (ppp - ppm)) + K.gxc * (p0p - p0m);
     const T gy = K.gye * ((ppm - pmm) + (ppp - pmp)) + K.gyc * (pp0 - pm0);
     const T lap = K.lcc * p00 + K.lex * (p0m + p0p) + K.ley * (pm0 + pp0) + K.lck * ((pmm + pmp) + (ppm + ppp));
-    double* mup = obs;
-    double* mua = obs + n;
-    double* jz = obs + 6 * n;
-    double* jn = obs + 8 * n;
-    mup[i] = (double)(p00 * (K.a + p00 * (K.b + p00 * K.c)) - K.kappa * lap);
-    mua[i] = (double)(K.lam * (gy * gy + gx * gx));
+    mup = p00 * (K.a + p00 * (K.b + p00 * K.c)) - K.kappa * lap;
+    mua = K.lam * (gy * gy + gx * gx);
     const T zl = K.zeta * lap;
-    jz[i] = (double)(zl * gy);
-    jz[n + i] = (double)(zl * gx);
-    T xy = 0, xx = 0;
-    if (p.noise) {
-        xy = (T)p.noise[i];
-        xx = (T)p.noise[n + i];
-    } else if (K.amp != T(0)) {
-        const uint64_t step = (uint64_t)(*p.step_base);
-        xy = normal1<T>(r, c, step, 0, p.key0, p.key1);
-        xx = normal1<T>(r, c, step, 1, p.key0, p.key1);
+    jzy = zl * gy;
+    jzx = zl * gx;
+}
+
+// Observables of one kind (AMB_OBS_*), computed only as far as that kind needs:
+// obs_local_kernel writes mu_p / mu_a into `tmp` ([mu_p | mu_a], n doubles each) when a
+// current needs their derivatives, and the scalar kinds straight into `out`;
+// obs_current_kernel forms the requested current into `out` ([y | x], 2n doubles) as
+// ((J_zeta + J_lambda) + J_p) + J_noise over the terms the kind includes.
+enum ObsTerm { kTermP = 1, kTermL = 2, kTermZ = 4, kTermN = 8 };
+
+__host__ __device__ inline int obs_terms(int which)
+{
+    switch (which) {
+    case 3: return kTermP | kTermL | kTermZ | kTermN;  // AMB_OBS_J
+    case 4: return kTermP | kTermL | kTermZ;           // AMB_OBS_J_DETERMINISTIC
+    case 5: return kTermP;                             // AMB_OBS_J_PASSIVE
+    case 6: return kTermL | kTermZ;                    // AMB_OBS_J_ACTIVE
+    case 7: return kTermL;                             // AMB_OBS_J_LAMBDA
+    case 8: return kTermZ;                             // AMB_OBS_J_CIRCULATING
+    case 9: return kTermN;                             // AMB_OBS_J_RANDOM
+    default: return 0;                                 // scalars
     }
-    jn[i] = (double)(K.amp * xy);
-    jn[n + i] = (double)(K.amp * xx);
 }
 
 template <typename T>
-__global__ __launch_bounds__(kThreads) void obs_current_kernel(const StepArgs<T> p, double* obs)
+__global__ __launch_bounds__(kThreads) void obs_local_kernel(const StepArgs<T> p, int which, double* tmp,
+                                                             double* out)
+{
+    const int ny = p.ny, nx = p.nx;
+    const size_t n = (size_t)ny * nx;
+    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const int r = (int)(i / nx), c = (int)(i - (size_t)r * nx);
+    T mup, mua, jzy, jzx;
+    local_terms(p, r, c, mup, mua, jzy, jzx);
+    const int terms = obs_terms(which);
+    if (terms & kTermP) tmp[i] = (double)mup;
+    if (terms & kTermL) tmp[n + i] = (double)mua;
+    if (which == 0) out[i] = (double)mup + (double)mua;  // AMB_OBS_MU
+    else if (which == 1) out[i] = (double)mup;          // AMB_OBS_MU_PASSIVE
+    else if (which == 2) out[i] = (double)mua;          // AMB_OBS_MU_ACTIVE
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void obs_current_kernel(const StepArgs<T> p, int which, const double* tmp,
+                                                               double* out)
 {
     const int ny = p.ny, nx = p.nx;
     const size_t n = (size_t)ny * nx;
@@ -228,43 +252,80 @@ __global__ __launch_bounds__(kThreads) void obs_current_kernel(const StepArgs<T>
     if (i >= n) return;
     const int r = (int)(i / nx), c = (int)(i - (size_t)r * nx);
     const StepCoef<T>& K = p.k;
-    const double* mup = obs;
-    const double* mua = obs + n;
-    double* jp = obs + 2 * n;
-    double* jl = obs + 4 * n;
-    // J_p = -D8 mu_p (src/integrator.cu:171-172); K.zx = 1/dx, K.zy = 1/dy.
-    T dy8 = 0, dx8 = 0;
-#pragma unroll
-    for (int k = 1; k <= 4; ++k) {
-        dy8 += (T)kD8[k] * ((T)ld(mup, ny, nx, r + k, c) - (T)ld(mup, ny, nx, r - k, c));
-        dx8 += (T)kD8[k] * ((T)ld(mup, ny, nx, r, c + k) - (T)ld(mup, ny, nx, r, c - k));
+    const int terms = obs_terms(which);
+    double zy = 0.0, zx = 0.0, ly = 0.0, lx = 0.0, py = 0.0, px = 0.0, ny_ = 0.0, nx_ = 0.0;
+    if (terms & kTermZ) {
+        T mup, mua, jzy, jzx;
+        local_terms(p, r, c, mup, mua, jzy, jzx);
+        zy = (double)jzy;
+        zx = (double)jzx;
     }
-    jp[i] = (double)(-K.zy * dy8);
-    jp[n + i] = (double)(-K.zx * dx8);
-    // J_lambda = -G3 mu_a (src/integrator.cu:174-175)
-    const T amm = (T)ld(mua, ny, nx, r - 1, c - 1), am0 = (T)ld(mua, ny, nx, r - 1, c), amp_ = (T)ld(mua, ny, nx, r - 1, c + 1);
-    const T a0m = (T)ld(mua, ny, nx, r, c - 1), a0p = (T)ld(mua, ny, nx, r, c + 1);
-    const T apm = (T)ld(mua, ny, nx, r + 1, c - 1), ap0 = (T)ld(mua, ny, nx, r + 1, c), app = (T)ld(mua, ny, nx, r + 1, c + 1);
-    const T gx = K.gxe * ((amp_ - amm) + (app - apm)) + K.gxc * (a0p - a0m);
-    const T gy = K.gye * ((apm - amm) + (app - amp_)) + K.gyc * (ap0 - am0);
-    jl[i] = (double)(-gy);
-    jl[n + i] = (double)(-gx);
+    if (terms & kTermP) {
+        // J_p = -D8 mu_p (src/integrator.cu:171-172); K.zx = 1/dx, K.zy = 1/dy.
+        const double* mp = tmp;
+        T dy8 = 0, dx8 = 0;
+#pragma unroll
+        for (int k = 1; k <= 4; ++k) {
+            dy8 += (T)kD8[k] * ((T)ld(mp, ny, nx, r + k, c) - (T)ld(mp, ny, nx, r - k, c));
+            dx8 += (T)kD8[k] * ((T)ld(mp, ny, nx, r, c + k) - (T)ld(mp, ny, nx, r, c - k));
+        }
+        py = (double)(-K.zy * dy8);
+        px = (double)(-K.zx * dx8);
+    }
+    if (terms & kTermL) {
+        // J_lambda = -G3 mu_a (src/integrator.cu:174-175)
+        const double* ma = tmp + n;
+        const T amm = (T)ld(ma, ny, nx, r - 1, c - 1), am0 = (T)ld(ma, ny, nx, r - 1, c),
+                amp_ = (T)ld(ma, ny, nx, r - 1, c + 1);
+        const T a0m = (T)ld(ma, ny, nx, r, c - 1), a0p = (T)ld(ma, ny, nx, r, c + 1);
+        const T apm = (T)ld(ma, ny, nx, r + 1, c - 1), ap0 = (T)ld(ma, ny, nx, r + 1, c),
+                app = (T)ld(ma, ny, nx, r + 1, c + 1);
+        const T gx = K.gxe * ((amp_ - amm) + (app - apm)) + K.gxc * (a0p - a0m);
+        const T gy = K.gye * ((apm - amm) + (app - amp_)) + K.gyc * (ap0 - am0);
+        ly = (double)(-gy);
+        lx = (double)(-gx);
+    }
+    if (terms & kTermN) {
+        T xy = 0, xx = 0;
+        if (p.noise) {
+            xy = (T)p.noise[i];
+            xx = (T)p.noise[n + i];
+        } else if (K.amp != T(0)) {
+            const uint64_t step = (uint64_t)(*p.step_base);
+            xy = normal1<T>(r, c, step, 0, p.key0, p.key1);
+            xx = normal1<T>(r, c, step, 1, p.key0, p.key1);
+        }
+        ny_ = (double)(K.amp * xy);
+        nx_ = (double)(K.amp * xx);
+    }
+    out[i] = ((zy + ly) + py) + ny_;
+    out[n + i] = ((zx + lx) + px) + nx_;
 }
 
 template <typename T>
-static hipError_t launch_obs(const StepArgs<T>& a, double* obs, hipStream_t s)
+static hipError_t launch_obs(const StepArgs<T>& a, int which, double* tmp, double* out, hipStream_t s)
 {
     const size_t n = (size_t)a.ny * a.nx;
     const unsigned blocks = (unsigned)((n + kThreads - 1) / kThreads);
-    hipLaunchKernelGGL(obs_local_kernel<T>, dim3(blocks), dim3(kThreads), 0, s, a, obs);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(obs_current_kernel<T>, dim3(blocks), dim3(kThreads), 0, s, a, obs);
+    const int terms = obs_terms(which);
+    if (which <= 2 || (terms & (kTermP | kTermL))) {
+        hipLaunchKernelGGL(obs_local_kernel<T>, dim3(blocks), dim3(kThreads), 0, s, a, which, tmp, out);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess || which <= 2) return e;
+    }
+    hipLaunchKernelGGL(obs_current_kernel<T>, dim3(blocks), dim3(kThreads), 0, s, a, which,
+                       static_cast<const double*>(tmp), out);
     return hipGetLastError();
 }
 
-hipError_t launch_observables_f32(const StepArgs<float>& a, double* obs, hipStream_t s) { return launch_obs(a, obs, s); }
-hipError_t launch_observables_f64(const StepArgs<double>& a, double* obs, hipStream_t s) { return launch_obs(a, obs, s); }
+hipError_t launch_observable_f32(const StepArgs<float>& a, int which, double* tmp, double* out, hipStream_t s)
+{
+    return launch_obs(a, which, tmp, out, s);
+}
+hipError_t launch_observable_f64(const StepArgs<double>& a, int which, double* tmp, double* out, hipStream_t s)
+{
+    return launch_obs(a, which, tmp, out, s);
+}
 
 // ---- FP64 mass and finite check ----------------------------------------------------
 
@@ -321,6 +382,41 @@ __global__ __launch_bounds__(kThreads) void reduce_final_kernel(const double* pa
         sum[0] = ssum[0];
         bad[0] = sbad[0];
     }
+}
+
+// Partials of the fused finite check: entry i = (sum, count) of tile i.  Each thread
+// sums a fixed stride of entries in index order, then a fixed tree.
+__global__ __launch_bounds__(kThreads) void reduce_tiles_kernel(const double* part, int64_t n, double* sum,
+                                                                unsigned long long* bad)
+{
+    __shared__ double ssum[kThreads];
+    __shared__ double sbad[kThreads];
+    double s = 0.0, b = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += kThreads) {
+        s += part[2 * i];
+        b += part[2 * i + 1];
+    }
+    ssum[threadIdx.x] = s;
+    sbad[threadIdx.x] = b;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            ssum[threadIdx.x] += ssum[threadIdx.x + w];
+            sbad[threadIdx.x] += sbad[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        sum[0] = ssum[0];
+        bad[0] = (unsigned long long)sbad[0];
+    }
+}
+
+hipError_t launch_reduce_tiles(const double* part, int64_t ntiles, double* sum, unsigned long long* bad,
+                               hipStream_t s)
+{
+    hipLaunchKernelGGL(reduce_tiles_kernel, dim3(1), dim3(kThreads), 0, s, part, ntiles, sum, bad);
+    return hipGetLastError();
 }
 
 template <typename T>

@@ -35,7 +35,9 @@ struct Lds {
     static constexpr int off_qx = off_mua + AW * AH;
     static constexpr int off_qy = off_qx + QW * QH;
     static constexpr int total = off_qy + QW * QH;
-    static constexpr size_t bytes = sizeof(T) * (size_t)total;
+    // per-wave FP64 partials of the fused mass / finite check (16 bytes per wave)
+    static constexpr size_t red_off = (sizeof(T) * (size_t)total + 15) / 16 * 16;
+    static constexpr size_t bytes = red_off + 16 * (size_t)(NT / 64);
     static_assert(NT % TW == 0, "tile width must divide the workgroup");
     static_assert((TW * TH) % NT == 0, "tile must be a multiple of the workgroup");
     static_assert(NT >= TW + 4, "workgroup must cover one stage-A row of the box");
@@ -152,7 +154,7 @@ template <typename T, int TW, int TH, int NT, int NOISE>
 __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in, T* __restrict__ out, const int ny,
                                           const int nx, const int nx4, const int x0, const int y0,
                                           const StepCoef<T>& K, const uint32_t key0, const uint32_t key1,
-                                          const uint64_t step, const double* noise)
+                                          const uint64_t step, const double* noise, double* const red_out)
 {
     using L = Lds<T, TW, TH, NT>;
     T* const s_phi = sm;
@@ -473,6 +475,7 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
             for (int k = 0; k < R + 8; ++k) ycol[k] = s_jy[(r0 + k) * L::YW + c];
         }
     }
+    T vout[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const int r = r0 + k;
@@ -512,8 +515,45 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
         acc = fma(K.gxe, hx[k] + hx[k + 2], acc);
         acc += sy[k + 2] - sy[k];
         const T v = fma(-K.dt, acc, phi_c[k]);
+        vout[k] = v;
         if (interior || (y0 + r < ny && x0 + c < nx)) *orow = v;
         orow += nx;
+    }
+
+    // ---- fused finite check and FP64 mass (the last step of a run): the tile's sum of
+    // finite new values and its count of non-finite ones, reduced in a fixed order (lane
+    // butterfly, then waves in index order), so the result does not depend on timing.
+    if (red_out != nullptr) {
+        double s = 0.0, bad = 0.0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            if (interior || (y0 + r0 + k < ny && x0 + c < nx)) {
+                const double d = (double)vout[k];
+                if (isfinite(d)) s += d;
+                else bad += 1.0;
+            }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            s += __shfl_xor(s, off, 64);
+            bad += __shfl_xor(bad, off, 64);
+        }
+        double* const red = reinterpret_cast<double*>(reinterpret_cast<unsigned char*>(sm) + L::red_off);
+        if ((tid & 63) == 0) {
+            red[2 * (tid >> 6)] = s;
+            red[2 * (tid >> 6) + 1] = bad;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double ts = 0.0, tb = 0.0;
+#pragma unroll
+            for (int w = 0; w < NT / 64; ++w) {
+                ts += red[2 * w];
+                tb += red[2 * w + 1];
+            }
+            red_out[0] = ts;
+            red_out[1] = tb;
+        }
     }
 }
 

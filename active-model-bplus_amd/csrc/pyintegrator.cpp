@@ -192,6 +192,14 @@ class Integrator {
         return py::make_tuple(s, bad);
     }
 
+    py::tuple last_mass() const
+    {
+        double s = 0;
+        int64_t bad = 0;
+        check(amb_last_mass(h_, &s, &bad));
+        return py::make_tuple(s, bad);
+    }
+
     void set_noise(const py::array_t<double, py::array::c_style | py::array::forcecast>& noise)
     {
         const int64_t per = 2 * (int64_t)ny_ * (int64_t)nx_;
@@ -453,6 +461,14 @@ class Ensemble {
         return py::make_tuple(s, bad);
     }
 
+    py::tuple last_mass() const
+    {
+        double s = 0;
+        int64_t bad = 0;
+        check(amb_batch_last_mass(b_, &s, &bad));
+        return py::make_tuple(s, bad);
+    }
+
     std::vector<int> tiles() const
     {
         std::vector<int> t(shapes_.size());
@@ -564,6 +580,8 @@ PYBIND11_MODULE(integrator, m)
         .def_property_readonly("seed", [](const Integrator& s) { return s.options()["seed"]; })
         .def_property_readonly("precision", [](const Integrator& s) { return s.options()["precision"]; })
         .def("mass", &Integrator::mass, "(FP64 sum of the field, number of non-finite sites)")
+        .def("last_mass", &Integrator::last_mass,
+             "(FP64 sum, number of non-finite sites) found by the last run, fused into its last step")
         .def("run", &Integrator::run, py::arg("nsteps"))
         .def("run_timed", &Integrator::run_timed, py::arg("nsteps"),
              "run(nsteps) returning the device milliseconds of the steps (HIP events on the integrator's stream)")
@@ -604,6 +622,8 @@ PYBIND11_MODULE(integrator, m)
         .def("run_timed", &Ensemble::run_timed, py::arg("nsteps"))
         .def("field", &Ensemble::field, py::arg("index"))
         .def("mass", &Ensemble::mass, py::arg("index"))
+        .def("last_mass", &Ensemble::last_mass,
+             "(FP64 sum over all realisations, number of non-finite sites) found by the last run")
         .def_property_readonly("tiles", &Ensemble::tiles)
         .def("set_tiles", &Ensemble::set_tiles, py::arg("tiles"))
         .def("autotune", &Ensemble::autotune, py::arg("reps") = 4)

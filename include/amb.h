@@ -109,6 +109,13 @@ int amb_get_options(const amb_handle* h, amb_options* options);
  * Replaces kernel::check_finite (src/integrator.cu:265-272) and adds the mass. */
 int amb_mass(amb_handle* h, double* sum, int64_t* nonfinite);
 
+/* The same two numbers as found by the last amb_run / amb_run_timed: with nsteps > 0
+ * they come from the run's last step launch, where every tile adds up its new values
+ * (FP64, fixed order) and counts the non-finite ones; one small fixed-order kernel then
+ * sums the tiles' partials.  Replaces kernel::check_finite's separate pass over the
+ * field after the step loop (src/integrator.cu:264-272,525-537). */
+int amb_last_mass(amb_handle* h, double* sum, int64_t* nonfinite);
+
 /* Test-only: explicit unscaled normals for the next `nsteps` steps
  * (nsteps * 2 * ny * nx doubles, step-major, each step [xi_y | xi_x]) replace the
  * Philox stream until they are used up; the caller's buffer is copied.  Lets a
@@ -157,6 +164,9 @@ int amb_batch_run(amb_batch* b, int64_t nsteps);
 int amb_batch_run_timed(amb_batch* b, int64_t nsteps, double* device_ms);
 int amb_batch_get_field(amb_batch* b, int64_t index, double* out);
 int amb_batch_mass(amb_batch* b, int64_t index, double* sum, int64_t* nonfinite);
+/* Total FP64 sum over all realisations and non-finite count found by the last
+ * amb_batch_run, fused into its last step launch as for amb_last_mass. */
+int amb_batch_last_mass(amb_batch* b, double* sum, int64_t* nonfinite);
 
 /* Tile shape (index from amb_tile_count) per realisation; realisations sharing a
  * shape share one launch per step. */
