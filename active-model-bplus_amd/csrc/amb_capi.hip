@@ -351,7 +351,14 @@ int reduce_now(amb_handle* h, double* sum, int64_t* bad)
 // field is read in full.
 int finish_run(amb_handle* h, int64_t nsteps, double* sum, int64_t* bad)
 {
-    if (nsteps == 0) return reduce_now(h, sum, bad);
+    if (nsteps == 0) {
+        const int rc = reduce_now(h, sum, bad);
+        if (rc == AMB_OK) {
+            h->last_sum = *sum;
+            h->last_bad = *bad;
+        }
+        return rc;
+    }
     hipError_t e = launch_reduce_tiles(h->d_tile_part, tiles_of(h, h->tile), h->d_sum, h->d_bad, h->stream);
     if (e != hipSuccess) return fail_hip(e, "reduction launch");
     if ((e = hipMemcpyAsync(h->h_sum, h->d_sum, sizeof(double), hipMemcpyDeviceToHost, h->stream)) != hipSuccess)

@@ -205,19 +205,12 @@ def bench_batched(args):
     """C4: 512 realisations, side in {64,128,256,512}, lambda~U[-2,2], zeta~U[-5,0],
     phi0~U[-0.6,0.6], T in {0,0.05,0.1}; batched vs sequential plain runs, on the <=256^2
     subset and on all realisations (plain times extrapolated from --steps/10 steps each)."""
-    from activemodelbplus.integrator import Ensemble, Integrator, Model, Stencil
+    from activemodelbplus.ensembles import random_ensemble
+    from activemodelbplus.integrator import Ensemble, Integrator
 
-    rng = np.random.default_rng(4)
     count = args.realisations
-    sides = rng.choice([64, 128, 256, 512], size=count)
-    lam = rng.uniform(-2, 2, size=count)
-    zeta = rng.uniform(-5, 0, size=count)
-    phi0 = rng.uniform(-0.6, 0.6, size=count)
-    temp = rng.choice([0.0, 0.05, 0.1], size=count)
-    fields = [phi0[i] + 0.01 * rng.normal(size=(sides[i], sides[i])) for i in range(count)]
-    models = [Model(-0.25, 0, 0.25, 1, lam[i], zeta[i], temp[i]) for i in range(count)]
-    seeds = [(4 << 32) + i for i in range(count)]
-    st = Stencil(5e-3, 1, 1)
+    fields, models, seeds, st, params = random_ensemble(count)
+    sides = params["side"]
     small = [i for i in range(count) if sides[i] <= 256]
     large = [i for i in range(count) if sides[i] > 256]
 

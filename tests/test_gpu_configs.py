@@ -1,0 +1,247 @@
+"""GPU parity at the benchmark configurations' own shapes, against the CPU oracle.
+
+  * 4096^2 noisy field (lamb = 0.5, zeta = -1, T = 0.05, dt = 5e-3, phi = 0.05 N(0,1),
+    seed 3): FP64 against the oracle for 10^2 steps with the Philox stream on both
+    sides (<= 1e-10 relative max-norm), FP32 against the same FP64 oracle field (bound
+    stated below), and FP32 over its full 2*10^4 steps (finite, FP64 mass drift bound).
+  * 1024^2 noisy field (lamb = -1, zeta = -4, T = 0.1, phi = -0.4 + 0.01 N(0,1), seed 2):
+    FP64 against the oracle for 10^3 steps (<= 1e-10) and FP32 against it (stated bound).
+  * The 512-member batched ensemble (sides 64..512, activemodelbplus.ensembles): 10^3
+    steps, every member finite with its FP64 mass kept; two members of every side equal
+    plain integrators bit for bit; one member of every side in FP64 batched mode against
+    the oracle (<= 1e-10).
+  * T > 0 FP64 parity for 10^3 steps at 64^2, with one shared noise buffer and with the
+    Philox stream.
+  * The reference's PhaseSeparationTest at its own shape (test/test_integrator.cc:230-251):
+    512 x 256, dy = 1.25, 10^5 steps, max and min within 0.1 of +-1.
+  * The fused finite check: the mass found inside the last step of a run equals a
+    separate full read of the field.
+
+Every FP32 bound is measured against the FP64 oracle and printed with the test.
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def relmax(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+@pytest.fixture(scope="module")
+def amb():
+    import activemodelbplus.integrator as m
+    return m
+
+
+def threads(oracle):
+    return max(1, min(16, oracle.max_threads()))
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+C3_STENCIL, C3_MODEL = (5e-3, 1, 1), (-0.25, 0, 0.25, 1, 0.5, -1, 0.05)
+C2_STENCIL, C2_MODEL = (5e-3, 1, 1), (-0.25, 0, 0.25, 1, -1, -4, 0.1)
+
+
+def c3_field():
+    return 0.05 * np.random.default_rng(3).normal(size=(4096, 4096))
+
+
+def c2_field():
+    return -0.4 + 0.01 * np.random.default_rng(2).normal(size=(1024, 1024))
+
+
+@pytest.mark.timeout(900)
+def test_4096_fp64_and_fp32_against_oracle(amb, oracle):
+    """4096^2, 10^2 noisy steps: FP64 <= 1e-10, FP32 <= 1e-5 relative max-norm against
+    the FP64 oracle (measured on MI355X: 6.1e-16 and 4.9e-6; the FP32 step rounds every
+    intermediate to 24 bits, and 100 steps of the stiff 17-tap operator amplify it)."""
+    phi = c3_field()
+    say("4096^2 noisy: oracle FP64, 100 steps")
+    ref = oracle.run(phi, C3_STENCIL, C3_MODEL, 100, seed=3, nthreads=threads(oracle))
+    for prec, bound in ((64, 1e-10), (32, 1e-5)):
+        say(f"4096^2 noisy: GPU FP{prec}, 100 steps")
+        sim = amb.Integrator(phi, amb.Stencil(*C3_STENCIL), amb.Model(*C3_MODEL), seed=3, precision=prec)
+        sim.run(100)
+        err = relmax(sim.field, ref)
+        say(f"4096^2 noisy, 100 steps: FP{prec} vs FP64 oracle rel max-norm {err:.3e} (bound {bound:g})")
+        assert err <= bound
+
+
+@pytest.mark.timeout(300)
+def test_4096_fp32_full_length(amb):
+    """4096^2 FP32 over its full 2*10^4 steps: finite, and the FP64 sum of the FP32 field
+    drifts by less than 1e-9 per site per step (each update is rounded to 24 bits)."""
+    phi = c3_field()
+    sim = amb.Integrator(phi, amb.Stencil(*C3_STENCIL), amb.Model(*C3_MODEL), seed=3, precision=32)
+    m0 = float(phi.astype(np.float32).astype(np.float64).sum())
+    say("4096^2 FP32: 20000 steps")
+    sim.run(20_000)
+    s, bad = sim.last_mass()
+    drift = (s - m0) / phi.size / 20_000
+    say(f"4096^2 FP32, 2e4 steps: non-finite {bad}, mass drift {s - m0:.3e} ({drift:.2e} per site per step)")
+    assert bad == 0
+    assert abs(drift) < 1e-9
+
+
+@pytest.mark.timeout(600)
+def test_1024_fp64_and_fp32_against_oracle(amb, oracle):
+    """1024^2, 10^3 noisy steps: FP64 <= 1e-10; FP32 <= 2e-5 relative max-norm against
+    the FP64 oracle."""
+    phi = c2_field()
+    say("1024^2 noisy: oracle FP64, 1000 steps")
+    ref = oracle.run(phi, C2_STENCIL, C2_MODEL, 1000, seed=2, nthreads=threads(oracle))
+    for prec, bound in ((64, 1e-10), (32, 2e-5)):
+        say(f"1024^2 noisy: GPU FP{prec}, 1000 steps")
+        sim = amb.Integrator(phi, amb.Stencil(*C2_STENCIL), amb.Model(*C2_MODEL), seed=2, precision=prec)
+        sim.run(1000)
+        err = relmax(sim.field, ref)
+        say(f"1024^2 noisy, 1e3 steps: FP{prec} vs FP64 oracle rel max-norm {err:.3e} (bound {bound:g})")
+        assert err <= bound
+        s, bad = sim.last_mass()
+        assert bad == 0 and abs(s - phi.sum()) / phi.size < (1e-12 if prec == 64 else 1e-6)
+
+
+@pytest.mark.timeout(600)
+def test_512_member_ensemble(amb, oracle):
+    """The 512-member ensemble, 10^3 FP32 steps: all finite, FP64 mass per member kept to
+    1e-6 per site; two members per side bit-equal to plain runs; one member per side in
+    FP64 batched mode against the oracle to 1e-10."""
+    from activemodelbplus.ensembles import random_ensemble
+    fields, models, seeds, st, params = random_ensemble(512)
+    sides = params["side"]
+    say("512-member ensemble FP32: autotune, 1000 steps")
+    ens = amb.Ensemble(fields, st, models, seeds, precision=32)
+    ens.autotune(2)
+    ens.run(1000)
+    total, bad = ens.last_mass()
+    assert bad == 0
+    m0 = [float(f.astype(np.float32).astype(np.float64).sum()) for f in fields]
+    assert abs(total - sum(m0)) / sum(f.size for f in fields) < 1e-6
+    worst = 0.0
+    for i in range(512):
+        s, b = ens.mass(i)
+        assert b == 0, i
+        worst = max(worst, abs(s - m0[i]) / fields[i].size)
+    say(f"512 members, 1e3 steps: worst FP64 mass drift per site {worst:.2e}")
+    assert worst < 1e-6
+    picks = [int(i) for side in (64, 128, 256, 512) for i in np.flatnonzero(sides == side)[:2]]
+    for i in picks:
+        sim = amb.Integrator(fields[i], st, models[i], seed=seeds[i], precision=32)
+        sim.run(1000)
+        np.testing.assert_array_equal(ens.field(i), sim.field, err_msg=f"member {i}, side {sides[i]}")
+    # FP64 batched members against the oracle
+    one = [int(np.flatnonzero(sides == side)[0]) for side in (64, 128, 256, 512)]
+    say(f"FP64 batched members {one}: 1000 steps vs oracle")
+    ens64 = amb.Ensemble([fields[i] for i in one], st, [models[i] for i in one], [seeds[i] for i in one],
+                         precision=64)
+    ens64.run(1000)
+    for k, i in enumerate(one):
+        ref = oracle.run(fields[i], st.as_tuple(), models[i].as_tuple(), 1000, seed=seeds[i],
+                         nthreads=threads(oracle))
+        err = relmax(ens64.field(k), ref)
+        say(f"member {i} (side {sides[i]}, T {params['T'][i]:g}): FP64 batched vs oracle {err:.3e}")
+        assert err <= 1e-10
+
+
+def test_64_fp64_noise_1000_steps(amb, oracle):
+    """T > 0, FP64, 64^2, 10^3 steps: one shared buffer of normals on both sides, and
+    the Philox stream on both sides; both <= 1e-10."""
+    rng = np.random.default_rng(21)
+    phi = -0.4 + 0.01 * rng.normal(size=(64, 64))
+    st, md = C2_STENCIL, C2_MODEL
+    buf = rng.normal(size=(1000, 2, 64, 64))
+    sim = amb.Integrator(phi, amb.Stencil(*st), amb.Model(*md), seed=9)
+    sim.set_noise(buf)
+    sim.run(1000)
+    ref = oracle.run(phi, st, md, 1000, noise_buf=buf)
+    err = relmax(sim.field, ref)
+    say(f"64^2 FP64 shared noise buffer, 1e3 steps: {err:.3e}")
+    assert err <= 1e-10
+    sim = amb.Integrator(phi, amb.Stencil(*st), amb.Model(*md), seed=31337, timestep=5)
+    sim.run(1000)
+    ref = oracle.run(phi, st, md, 1000, seed=31337, step0=5)
+    err = relmax(sim.field, ref)
+    say(f"64^2 FP64 Philox, 1e3 steps: {err:.3e}")
+    assert err <= 1e-10
+
+
+def test_phase_separation_reference_shape(amb):
+    """test/test_integrator.cc:230-251: 512 x 256 (nx x ny), dy = 1.25, dt = 1e-2, passive
+    model with binodals +-1, phi = 0.1 U[-1, 1] minus its mean, 10^5 steps: the field's
+    max and min are within 0.1 of +1 and -1."""
+    rng = np.random.default_rng(23)
+    phi = 0.1 * rng.uniform(-1, 1, size=(256, 512))
+    phi -= phi.mean()
+    sim = amb.Integrator(phi, amb.Stencil(1e-2, 1, 1.25), amb.Model(-0.25, 0, 0.25, 1, 0, 0, 0))
+    sim.run(100_000)
+    f = sim.field
+    say(f"phase separation 512x256, 1e5 steps: max {f.max():.4f} min {f.min():.4f}")
+    assert abs(f.max() - 1) < 0.1 and abs(f.min() + 1) < 0.1
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("nsteps", [0, 1, 31, 32, 33, 100])
+def test_fused_mass_equals_full_read(amb, precision, nsteps):
+    """The mass and non-finite count found inside the last step of run(n) equal a
+    separate full read of the field (1e-12 relative; both are FP64 sums, in different
+    orders)."""
+    rng = np.random.default_rng(nsteps)
+    phi = 0.3 + 0.05 * rng.normal(size=(200, 136))
+    sim = amb.Integrator(phi, amb.Stencil(*C2_STENCIL), amb.Model(*C2_MODEL), seed=5, precision=precision)
+    sim.run(nsteps)
+    fused, bad = sim.last_mass()
+    full, bad_full = sim.mass()
+    assert bad == 0 and bad_full == 0
+    assert abs(fused - full) <= 1e-12 * abs(full)
+    # and after a tile change (other partial count)
+    sim.set_tile(len(amb.tile_shapes(precision)) - 1)
+    sim.run(nsteps + 3)
+    fused, _ = sim.last_mass()
+    full, _ = sim.mass()
+    assert abs(fused - full) <= 1e-12 * abs(full)
+
+
+def test_fused_check_counts_nonfinite(amb):
+    """A NaN planted in the field spreads; the fused check counts every non-finite site
+    (equal to the full read's count) and run() raises."""
+    phi = np.zeros((64, 64))
+    phi[10, 20] = np.nan
+    sim = amb.Integrator(phi, amb.Stencil(1e-2, 1, 1), amb.Model(-0.25, 0, 0.25, 1, 0, 0, 0))
+    with pytest.raises(amb.CudaException):
+        sim.run(3)
+    _, bad = sim.last_mass()
+    _, bad_full = sim.mass()
+    assert bad == bad_full and bad > 1
+
+
+def test_ensemble_fused_mass(amb):
+    rng = np.random.default_rng(3)
+    fields = [0.1 + 0.02 * rng.normal(size=(n, n)) for n in (48, 64, 100, 130)]
+    st = amb.Stencil(5e-3, 1, 1)
+    models = [amb.Model(-0.25, 0, 0.25, 1, 1, -2, t) for t in (0, 0.05, 0.1, 0)]
+    ens = amb.Ensemble(fields, st, models, [1, 2, 3, 4], precision=64)
+    for n in (5, 64, 0):
+        ens.run(n)
+        total, bad = ens.last_mass()
+        parts = sum(ens.mass(i)[0] for i in range(4))
+        assert bad == 0 and abs(total - parts) <= 1e-12 * abs(parts)
+
+
+def test_structure_factor_is_deterministic(amb):
+    """Two calls on the same field give the same bits (per-wave ordered binning)."""
+    rng = np.random.default_rng(7)
+    for precision in (64, 32):
+        sim = amb.Integrator(0.1 * rng.normal(size=(256, 192)), amb.Stencil(5e-3, 1, 1),
+                             amb.Model(-0.25, 0, 0.25, 1, 0, 0, 0), precision=precision)
+        edges = np.linspace(0, 3, 300)[1:]
+        a = sim.structure_factor_bins(edges)
+        b = sim.structure_factor_bins(edges)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+        assert int(a[1].sum()) == 256 * 192

@@ -145,8 +145,8 @@ def test_active_fp64_parity(amb, oracle, shape, stencil, model):
     rng = np.random.default_rng(4)
     phi = -0.2 + 0.1 * rng.normal(size=shape)
     sim = make(amb, phi, stencil, model, seed=3)
-    sim.run(300)
-    ref = oracle.run(phi, stencil, model, 300)
+    sim.run(1000)
+    ref = oracle.run(phi, stencil, model, 1000)
     assert relmax(sim.field, ref) <= 1e-10
 
 

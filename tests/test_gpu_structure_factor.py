@@ -8,7 +8,8 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("precision,shape,dx,dy", [(64, (256, 256), 1.0, 1.0), (64, (96, 160), 0.75, 1.25),
-                                                   (32, (128, 128), 1.0, 1.0), (32, (64, 200), 1.0, 0.5)])
+                                                   (32, (128, 128), 1.0, 1.0), (32, (64, 200), 1.0, 0.5),
+                                                   (64, (60, 75), 1.0, 1.0), (32, (33, 51), 1.0, 1.0)])
 def test_device_bins_match_numpy(precision, shape, dx, dy):
     import activemodelbplus.integrator as amb
     from activemodelbplus import trajectory as T
