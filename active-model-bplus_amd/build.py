@@ -65,7 +65,7 @@ def build_lib(verbose: bool = False) -> str:
         _run([_hipcc(), *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj], quiet=not verbose)
         return obj
 
-    with ThreadPoolExecutor(max_workers=min(5, len(HIP_SOURCES))) as ex:
+    with ThreadPoolExecutor(max_workers=min(6, len(HIP_SOURCES))) as ex:
         objs = list(ex.map(compile_one, HIP_SOURCES))
     tmp = LIBAMB + ".tmp"
     _run([_hipcc(), "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs,

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <new>
 #include <utility>
 #include <vector>
@@ -32,7 +33,9 @@ struct BatchDesc {
     StepCoef<T> k;
 };
 
-template <typename T, int TW, int TH, int NT>
+// NOISE is kNoiseNone or kNoisePhilox: members with and without noise run in separate
+// launches, so each kernel holds only its own stage-B windows in registers.
+template <typename T, int TW, int TH, int NT, int NOISE>
 __global__ __launch_bounds__(NT) void batch_step_kernel(const BatchDesc<T>* __restrict__ desc,
                                                               const int2* __restrict__ tiles, const T* in, T* out,
                                                               const long long* step_base, int step_off, double* part)
@@ -50,8 +53,155 @@ __global__ __launch_bounds__(NT) void batch_step_kernel(const BatchDesc<T>* __re
     const BatchDesc<T>& d = desc[t.x];
     const int ty = t.y / d.ntx, tx = t.y - ty * d.ntx;
     const uint64_t step = (uint64_t)(*step_base) + (uint64_t)step_off;
-    step_tile<T, TW, TH, NT, kNoiseRuntime>(reinterpret_cast<T*>(smem_raw), in + d.off, out + d.off, d.ny, d.nx,
-                                            d.nx4, tx * TW, ty * TH, d.k, d.key0, d.key1, step, nullptr, red);
+    step_tile<T, TW, TH, NT, NOISE>(reinterpret_cast<T*>(smem_raw), in + d.off, out + d.off, d.ny, d.nx, d.nx4,
+                                    tx * TW, ty * TH, d.k, d.key0, d.key1, step, nullptr, red);
+}
+
+// ---- resident members: one workgroup keeps a whole small realisation in LDS ------------
+//
+// For members whose two field buffers and one tile's scratch fit in kResidentLds bytes,
+// one launch advances `nsteps` steps with no global traffic in between: the workgroup
+// loads its member's field into LDS, applies step_tile to each tile of the member in turn
+// (reading one LDS buffer, writing the other; step_tile accepts any address space through
+// generic pointers), swaps the buffers after each step and writes the field back once.
+// No workgroup waits for another.  Same per-site arithmetic as every other path, so a
+// resident member's trajectory equals a plain integrator's bit for bit.  The last act of a
+// launch is the member's partial of the fused finite check (its FP64 sum and non-finite
+// count, reduced in a fixed order) at part[slot].
+constexpr size_t kResidentLds = 80 * 1024;
+
+template <typename T>
+__host__ __device__ constexpr size_t resident_field_bytes(int64_t n)
+{
+    return ((size_t)n * sizeof(T) + 15) / 16 * 16;
+}
+
+template <typename T, int TW, int TH, int NT, int NOISE>
+__global__ __launch_bounds__(NT) void resident_kernel(const BatchDesc<T>* __restrict__ desc,
+                                                      const int* __restrict__ members, const T* in, T* out,
+                                                      const long long* step_base, int nsteps, double* part,
+                                                      int64_t part_base)
+{
+    using L = Lds<T, TW, TH, NT>;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const BatchDesc<T>& d = desc[members[blockIdx.x]];
+    const int64_t n = (int64_t)d.ny * d.nx;
+    const size_t fb = resident_field_bytes<T>(n);
+    T* a = reinterpret_cast<T*>(smem_raw);
+    T* b = reinterpret_cast<T*>(smem_raw + fb);
+    T* const scratch = reinterpret_cast<T*>(smem_raw + 2 * fb);
+    const int tid = threadIdx.x;
+    for (int64_t i = tid; i < n; i += NT) a[i] = in[d.off + i];
+    __syncthreads();
+    const uint64_t step0 = (uint64_t)(*step_base);
+    const int ntx = (d.nx + TW - 1) / TW, nty = (d.ny + TH - 1) / TH;
+    for (int k = 0; k < nsteps; ++k) {
+        for (int t = 0; t < ntx * nty; ++t) {
+            const int ty = t / ntx, tx = t - ty * ntx;
+            step_tile<T, TW, TH, NT, NOISE>(scratch, a, b, d.ny, d.nx, d.nx4, tx * TW, ty * TH, d.k, d.key0, d.key1,
+                                            step0 + (uint64_t)k, nullptr, nullptr);
+            __syncthreads();  // the next tile's stage 0 overwrites what this one's stage B reads
+        }
+        T* const c = a;
+        a = b;
+        b = c;
+    }
+    double s = 0.0, bad = 0.0;
+    for (int64_t i = tid; i < n; i += NT) {
+        const T v = a[i];
+        out[d.off + i] = v;
+        const double dv = (double)v;
+        if (isfinite(dv)) s += dv;
+        else bad += 1.0;
+    }
+    // fixed-order block reduction: lane butterfly, then the waves in index order
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        s += __shfl_xor(s, off, 64);
+        bad += __shfl_xor(bad, off, 64);
+    }
+    double* const red = reinterpret_cast<double*>(reinterpret_cast<unsigned char*>(scratch) + L::red_off);
+    __syncthreads();
+    if ((tid & 63) == 0) {
+        red[2 * (tid >> 6)] = s;
+        red[2 * (tid >> 6) + 1] = bad;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double ts = 0.0, tb = 0.0;
+        for (int w = 0; w < NT / 64; ++w) {
+            ts += red[2 * w];
+            tb += red[2 * w + 1];
+        }
+        part[2 * (part_base + blockIdx.x)] = ts;
+        part[2 * (part_base + blockIdx.x) + 1] = tb;
+    }
+}
+
+template <typename T>
+struct ResidentShape;
+template <>
+struct ResidentShape<float> {
+    static constexpr int tw = 32, th = 32, nt = 256;
+};
+template <>
+struct ResidentShape<double> {
+    static constexpr int tw = 32, th = 16, nt = 256;
+};
+
+template <typename T>
+constexpr size_t resident_lds(int64_t n)
+{
+    using S = ResidentShape<T>;
+    return 2 * resident_field_bytes<T>(n) + Lds<T, S::tw, S::th, S::nt>::bytes;
+}
+
+// A member runs resident when its two buffers and one tile's scratch fit kResidentLds.
+static bool fits_resident(int precision, int64_t ny, int64_t nx)
+{
+    const int64_t n = ny * nx;
+    return precision == AMB_F32 ? resident_lds<float>(n) <= kResidentLds : resident_lds<double>(n) <= kResidentLds;
+}
+
+template <typename T, int NOISE>
+static hipError_t launch_resident_t(const void* desc, const int* members, int count, int64_t max_n, const void* in,
+                                    void* out, const long long* step, int nsteps, double* part, int64_t part_base,
+                                    hipStream_t s)
+{
+    using S = ResidentShape<T>;
+    hipLaunchKernelGGL((resident_kernel<T, S::tw, S::th, S::nt, NOISE>), dim3(count), dim3(S::nt),
+                       resident_lds<T>(max_n), s, static_cast<const BatchDesc<T>*>(desc), members,
+                       static_cast<const T*>(in), static_cast<T*>(out), step, nsteps, part, part_base);
+    return hipGetLastError();
+}
+
+static hipError_t launch_resident(int precision, int noise, const void* desc, const int* members, int count,
+                                  int64_t max_n, const void* in, void* out, const long long* step, int nsteps,
+                                  double* part, int64_t part_base, hipStream_t s)
+{
+    if (precision == AMB_F32)
+        return noise == kNoisePhilox
+                   ? launch_resident_t<float, kNoisePhilox>(desc, members, count, max_n, in, out, step, nsteps, part,
+                                                            part_base, s)
+                   : launch_resident_t<float, kNoiseNone>(desc, members, count, max_n, in, out, step, nsteps, part,
+                                                          part_base, s);
+    return noise == kNoisePhilox
+               ? launch_resident_t<double, kNoisePhilox>(desc, members, count, max_n, in, out, step, nsteps, part,
+                                                         part_base, s)
+               : launch_resident_t<double, kNoiseNone>(desc, members, count, max_n, in, out, step, nsteps, part,
+                                                       part_base, s);
+}
+
+template <typename T>
+static hipError_t raise_resident_lds_t()
+{
+    using S = ResidentShape<T>;
+    hipError_t e = hipFuncSetAttribute((const void*)resident_kernel<T, S::tw, S::th, S::nt, kNoiseNone>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kResidentLds);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)resident_kernel<T, S::tw, S::th, S::nt, kNoisePhilox>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kResidentLds);
+    return e;
 }
 
 template <typename T, int TW, int TH, int NT>
@@ -59,8 +209,12 @@ static hipError_t raise_batch_lds()
 {
     using L = Lds<T, TW, TH, NT>;
     if (L::bytes <= 65536) return hipSuccess;
-    return hipFuncSetAttribute((const void*)batch_step_kernel<T, TW, TH, NT>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)L::bytes);
+    hipError_t e = hipFuncSetAttribute((const void*)batch_step_kernel<T, TW, TH, NT, kNoiseNone>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)L::bytes);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)batch_step_kernel<T, TW, TH, NT, kNoisePhilox>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)L::bytes);
+    return e;
 }
 
 #define AMB_BRAISE_F32(i, tw, th, nt) if ((e = raise_batch_lds<float, tw, th, nt>()) != hipSuccess) return e;
@@ -72,28 +226,34 @@ static hipError_t prepare_batch_kernels()
     hipError_t e = hipSuccess;
     AMB_F32_TILES(AMB_BRAISE_F32)
     AMB_F64_TILES(AMB_BRAISE_F64)
-    return e;
+    if ((e = raise_resident_lds_t<float>()) != hipSuccess) return e;
+    return raise_resident_lds_t<double>();
 }
 
 template <typename T, int TW, int TH, int NT>
-static hipError_t launch_batch_one(const void* desc, const int2* tiles, int ntiles, const void* in, void* out,
-                                   const long long* step, int step_off, double* part, hipStream_t s)
+static hipError_t launch_batch_one(int noise, const void* desc, const int2* tiles, int ntiles, const void* in,
+                                   void* out, const long long* step, int step_off, double* part, hipStream_t s)
 {
     using L = Lds<T, TW, TH, NT>;
-    hipLaunchKernelGGL((batch_step_kernel<T, TW, TH, NT>), dim3(ntiles), dim3(NT), L::bytes, s,
-                       static_cast<const BatchDesc<T>*>(desc), tiles, static_cast<const T*>(in), static_cast<T*>(out),
-                       step, step_off, part);
+    const auto* d = static_cast<const BatchDesc<T>*>(desc);
+    if (noise == kNoisePhilox)
+        hipLaunchKernelGGL((batch_step_kernel<T, TW, TH, NT, kNoisePhilox>), dim3(ntiles), dim3(NT), L::bytes, s, d,
+                           tiles, static_cast<const T*>(in), static_cast<T*>(out), step, step_off, part);
+    else
+        hipLaunchKernelGGL((batch_step_kernel<T, TW, TH, NT, kNoiseNone>), dim3(ntiles), dim3(NT), L::bytes, s, d,
+                           tiles, static_cast<const T*>(in), static_cast<T*>(out), step, step_off, part);
     return hipGetLastError();
 }
 
 #define AMB_BCASE_F32(i, tw, th, nt) \
-    case i: return launch_batch_one<float, tw, th, nt>(desc, tiles, ntiles, in, out, step, step_off, part, s);
+    case i: return launch_batch_one<float, tw, th, nt>(noise, desc, tiles, ntiles, in, out, step, step_off, part, s);
 #define AMB_BCASE_F64(i, tw, th, nt) \
-    case i: return launch_batch_one<double, tw, th, nt>(desc, tiles, ntiles, in, out, step, step_off, part, s);
+    case i: return launch_batch_one<double, tw, th, nt>(noise, desc, tiles, ntiles, in, out, step, step_off, part, s);
 
-// part (nullable): the per-tile partials of the fused finite check, one entry per entry of
-// `tiles` (padding entries write zeros).
-static hipError_t launch_batch(int precision, int tile, const void* desc, const int2* tiles, int ntiles,
+// One step of the members listed in `tiles` (noise: kNoiseNone or kNoisePhilox, the same
+// for all of them).  part (nullable): the per-tile partials of the fused finite check,
+// one entry per entry of `tiles` (padding entries write zeros).
+static hipError_t launch_batch(int precision, int tile, int noise, const void* desc, const int2* tiles, int ntiles,
                                const void* in, void* out, const long long* step, int step_off, double* part,
                                hipStream_t s)
 {
@@ -119,6 +279,7 @@ constexpr int kBatchGraphSteps = 32;
 
 struct Group {
     int tile = 0;
+    int noise = 0;  // kNoiseNone or kNoisePhilox
     std::vector<int> members;
     int2* d_tiles = nullptr;
     int ntiles = 0;
@@ -147,6 +308,14 @@ struct amb_batch {
     int64_t tile_part_cap = 0;      // entries allocated in d_tile_part
     double last_sum = 0.0;          // total mass and non-finite count after the last run
     int64_t last_bad = 0;
+    // members kept in LDS by the resident kernel (when enabled and they fit)
+    bool resident_on = true;
+    std::vector<int> resident;      // noiseless members first, then the noisy ones
+    int64_t resident_quiet = 0;     // how many of them are noiseless
+    int* d_resident = nullptr;
+    int64_t resident_max_n = 0;
+    hipStream_t side = nullptr;     // second capture branch: the resident launch runs beside the tiled steps
+    hipEvent_t fork = nullptr, join = nullptr;
     double* d_part = nullptr;
     unsigned long long* d_bad_part = nullptr;
     double* d_sum = nullptr;
@@ -179,9 +348,14 @@ void release(amb_batch* b)
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     drop_graphs(b);
     free_groups(b);
+    if (b->side) (void)hipStreamSynchronize(b->side);
     for (void* p : {b->buf[0], b->buf[1], (void*)b->d_step, (void*)b->d_step_scratch, b->d_desc, (void*)b->d_part,
-                    (void*)b->d_bad_part, (void*)b->d_sum, (void*)b->d_bad, (void*)b->d_tile_part})
+                    (void*)b->d_bad_part, (void*)b->d_sum, (void*)b->d_bad, (void*)b->d_tile_part,
+                    (void*)b->d_resident})
         if (p) (void)hipFree(p);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->join) (void)hipEventDestroy(b->join);
+    if (b->side) (void)hipStreamDestroy(b->side);
     if (b->h_sum) (void)hipHostFree(b->h_sum);
     if (b->h_bad) (void)hipHostFree(b->h_bad);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
@@ -213,12 +387,35 @@ int rebuild_groups(amb_batch* b)
 {
     drop_graphs(b);
     free_groups(b);
-    std::map<int, std::vector<int>> by_tile;
-    for (size_t r = 0; r < b->desc.size(); ++r) by_tile[b->tile_of[r]].push_back((int)r);
+    std::map<std::pair<int, int>, std::vector<int>> by_tile;  // (tile, noise mode) -> members
+    std::vector<int> quiet, noisy;
+    b->resident_max_n = 0;
+    for (size_t r = 0; r < b->desc.size(); ++r) {
+        const amb_realisation& m = b->desc[r];
+        const int noise = m.model.T > 0.0 ? kNoisePhilox : kNoiseNone;
+        if (b->resident_on && fits_resident(b->precision, m.ny, m.nx)) {
+            (noise == kNoisePhilox ? noisy : quiet).push_back((int)r);
+            b->resident_max_n = std::max(b->resident_max_n, m.ny * m.nx);
+        } else {
+            by_tile[{b->tile_of[r], noise}].push_back((int)r);
+        }
+    }
+    b->resident = quiet;
+    b->resident.insert(b->resident.end(), noisy.begin(), noisy.end());
+    b->resident_quiet = (int64_t)quiet.size();
+    if (b->d_resident) (void)hipFree(b->d_resident);
+    b->d_resident = nullptr;
+    if (!b->resident.empty()) {
+        hipError_t e = hipMalloc(&b->d_resident, b->resident.size() * sizeof(int));
+        if (e == hipSuccess)
+            e = hipMemcpy(b->d_resident, b->resident.data(), b->resident.size() * sizeof(int), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return fail_hip(e, "resident member list upload");
+    }
     b->ntiles_total = 0;
     for (auto& kv : by_tile) {
         Group g;
-        g.tile = kv.first;
+        g.tile = kv.first.first;
+        g.noise = kv.first.second;
         g.members = kv.second;
         const std::vector<int2> list = tile_list(b, g.members, g.tile);
         g.ntiles = (int)list.size();
@@ -232,13 +429,14 @@ int rebuild_groups(amb_batch* b)
         }
         b->groups.push_back(std::move(g));
     }
-    if (b->ntiles_total > b->tile_part_cap) {
+    const int64_t nparts = b->ntiles_total + (int64_t)b->resident.size();  // resident partials after the tiles'
+    if (nparts > b->tile_part_cap) {
         if (b->d_tile_part) (void)hipFree(b->d_tile_part);
         b->d_tile_part = nullptr;
         b->tile_part_cap = 0;
-        const hipError_t e = hipMalloc(&b->d_tile_part, (size_t)b->ntiles_total * 2 * sizeof(double));
+        const hipError_t e = hipMalloc(&b->d_tile_part, (size_t)nparts * 2 * sizeof(double));
         if (e != hipSuccess) return fail_hip(e, "hipMalloc (tile partials)");
-        b->tile_part_cap = b->ntiles_total;
+        b->tile_part_cap = nparts;
     }
     return AMB_OK;
 }
@@ -282,7 +480,8 @@ int upload_descs(amb_batch* b)
 hipError_t launch_groups(amb_batch* b, int parity_in, const long long* step, int step_off, bool reduce)
 {
     for (const Group& g : b->groups) {
-        const hipError_t e = launch_batch(b->precision, g.tile, b->d_desc, g.d_tiles, g.ntiles, b->buf[parity_in & 1],
+        const hipError_t e = launch_batch(b->precision, g.tile, g.noise, b->d_desc, g.d_tiles, g.ntiles,
+                                          b->buf[parity_in & 1],
                                           b->buf[(parity_in + 1) & 1], step, step_off,
                                           reduce ? b->d_tile_part + 2 * g.part_base : nullptr, b->stream);
         if (e != hipSuccess) return e;
@@ -290,13 +489,41 @@ hipError_t launch_groups(amb_batch* b, int parity_in, const long long* step, int
     return hipSuccess;
 }
 
+// nsteps steps of every resident member in one launch on stream s, from buffer
+// `parity_in` to buffer parity_in + nsteps.
+hipError_t launch_resident_members(amb_batch* b, int parity_in, int nsteps, hipStream_t s)
+{
+    if (b->resident.empty() || nsteps <= 0) return hipSuccess;
+    const int64_t nq = b->resident_quiet, nn = (int64_t)b->resident.size() - nq;
+    hipError_t e = hipSuccess;
+    if (nq > 0)
+        e = launch_resident(b->precision, kNoiseNone, b->d_desc, b->d_resident, (int)nq, b->resident_max_n,
+                            b->buf[parity_in & 1], b->buf[(parity_in + nsteps) & 1], b->d_step, nsteps,
+                            b->d_tile_part, b->ntiles_total, s);
+    if (e == hipSuccess && nn > 0)
+        e = launch_resident(b->precision, kNoisePhilox, b->d_desc, b->d_resident + nq, (int)nn, b->resident_max_n,
+                            b->buf[parity_in & 1], b->buf[(parity_in + nsteps) & 1], b->d_step, nsteps,
+                            b->d_tile_part, b->ntiles_total + nq, s);
+    return e;
+}
+
+// The graph of kBatchGraphSteps steps: the resident members' one launch on a branch of
+// its own (fork from and join to the main stream), beside the tiled members' steps.
 int build_graph(amb_batch* b, int parity)
 {
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return fail_hip(e, "hipStreamBeginCapture");
+    const bool branch = !b->resident.empty();
+    if (branch) {
+        e = hipEventRecord(b->fork, b->stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(b->side, b->fork, 0);
+        if (e == hipSuccess) e = launch_resident_members(b, parity, kBatchGraphSteps, b->side);
+        if (e == hipSuccess) e = hipEventRecord(b->join, b->side);
+    }
     for (int k = 0; k < kBatchGraphSteps && e == hipSuccess; ++k)
         e = launch_groups(b, parity + k, b->d_step, k, k == kBatchGraphSteps - 1);
+    if (e == hipSuccess && branch) e = hipStreamWaitEvent(b->stream, b->join, 0);
     if (e == hipSuccess) e = launch_advance(b->d_step, kBatchGraphSteps, b->stream);
     const hipError_t e2 = hipStreamEndCapture(b->stream, &g);
     if (e != hipSuccess) {
@@ -327,6 +554,8 @@ int enqueue(amb_batch* b, int64_t nsteps)
             if ((e = hipGraphLaunch(b->graph[p], b->stream)) != hipSuccess) return fail_hip(e, "hipGraphLaunch");
         left -= reps * kBatchGraphSteps;
     }
+    if (left > 0 && (e = launch_resident_members(b, b->cur, (int)left, b->stream)) != hipSuccess)
+        return fail_hip(e, "resident launch");
     for (int k = 0; k < left; ++k)
         if ((e = launch_groups(b, b->cur + k, b->d_step, k, k == left - 1)) != hipSuccess)
             return fail_hip(e, "batched launch");
@@ -362,7 +591,8 @@ int reduce_range(amb_batch* b, int64_t off, int64_t n, double* sum, int64_t* bad
 int finish_run(amb_batch* b, int64_t nsteps, double* sum, int64_t* bad)
 {
     if (nsteps == 0) return reduce_range(b, 0, b->total, sum, bad);
-    hipError_t e = launch_reduce_tiles(b->d_tile_part, b->ntiles_total, b->d_sum, b->d_bad, b->stream);
+    hipError_t e = launch_reduce_tiles(b->d_tile_part, b->ntiles_total + (int64_t)b->resident.size(), b->d_sum,
+                                       b->d_bad, b->stream);
     if (e != hipSuccess) return fail_hip(e, "reduction launch");
     if ((e = hipMemcpyAsync(b->h_sum, b->d_sum, sizeof(double), hipMemcpyDeviceToHost, b->stream)) != hipSuccess)
         return fail_hip(e, "hipMemcpyAsync");
@@ -423,6 +653,9 @@ int amb_batch_create(const amb_realisation* desc, int64_t count, const double* c
     const size_t desc_bytes = (size_t)count * (precision == AMB_F32 ? sizeof(BatchDesc<float>) : sizeof(BatchDesc<double>));
     AMB_CHECK(prepare_batch_kernels());
     AMB_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    AMB_CHECK(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
+    AMB_CHECK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    AMB_CHECK(hipEventCreateWithFlags(&b->join, hipEventDisableTiming));
     AMB_CHECK(hipMalloc(&b->buf[0], (size_t)total * b->esize));
     AMB_CHECK(hipMalloc(&b->buf[1], (size_t)total * b->esize));
     AMB_CHECK(hipMalloc(&b->d_step, sizeof(long long)));
@@ -540,6 +773,23 @@ int amb_batch_mass(amb_batch* b, int64_t index, double* sum, int64_t* nonfinite)
     return AMB_OK;
 }
 
+int amb_batch_set_resident(amb_batch* b, int enable)
+{
+    clear_error();
+    if (!b) return fail(AMB_E_ARG, "batch must not be null");
+    hipError_t e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return fail_hip(e, "hipStreamSynchronize");
+    b->resident_on = enable != 0;
+    return rebuild_groups(b);
+}
+
+int amb_batch_resident_count(amb_batch* b, int64_t* count)
+{
+    if (!b || !count) return fail(AMB_E_ARG, "batch and count must not be null");
+    *count = (int64_t)b->resident.size();
+    return AMB_OK;
+}
+
 int amb_batch_last_mass(amb_batch* b, double* sum, int64_t* nonfinite)
 {
     clear_error();
@@ -581,15 +831,21 @@ int amb_batch_autotune(amb_batch* b, int reps, int* chosen)
     if (reps < 1) reps = 4;
     hipError_t e = hipStreamSynchronize(b->stream);
     if (e != hipSuccess) return fail_hip(e, "hipStreamSynchronize");
-    std::map<std::pair<int64_t, int64_t>, std::vector<int>> classes;
-    for (size_t r = 0; r < b->desc.size(); ++r) classes[{b->desc[r].ny, b->desc[r].nx}].push_back((int)r);
+    // classes of equal grid and noise mode (each noise mode has its own kernel)
+    std::map<std::tuple<int64_t, int64_t, int>, std::vector<int>> classes;
+    std::vector<bool> is_resident(b->desc.size(), false);
+    for (int r : b->resident) is_resident[r] = true;
+    for (size_t r = 0; r < b->desc.size(); ++r)
+        if (!is_resident[r])
+            classes[{b->desc[r].ny, b->desc[r].nx, b->desc[r].model.T > 0.0 ? kNoisePhilox : kNoiseNone}].push_back(
+                (int)r);
     void* saved = nullptr;
     if ((e = hipMalloc(&saved, (size_t)b->total * b->esize)) != hipSuccess) return fail_hip(e, "hipMalloc");
     e = hipMemcpy(saved, b->buf[b->cur], (size_t)b->total * b->esize, hipMemcpyDeviceToDevice);
     const long long zero = 0;
     if (e == hipSuccess) e = hipMemcpy(b->d_step_scratch, &zero, sizeof(long long), hipMemcpyHostToDevice);
     const int nt = tile_count(b->precision);
-    std::vector<int> best_tile(b->desc.size(), 0);
+    std::vector<int> best_tile = b->tile_of;  // resident members keep theirs (unused while resident)
     for (auto& kv : classes) {
         if (e != hipSuccess) break;
         float best = 1e30f;
@@ -614,7 +870,8 @@ int amb_batch_autotune(amb_batch* b, int reps, int* chosen)
             for (int k = 0; k <= reps && e == hipSuccess; ++k) {
                 if (k == 1) e = hipEventRecord(b->ev0, b->stream);
                 if (e == hipSuccess)
-                    e = launch_batch(b->precision, t, b->d_desc, d_tiles, (int)list.size(), b->buf[(b->cur + k) & 1],
+                    e = launch_batch(b->precision, t, std::get<2>(kv.first), b->d_desc, d_tiles, (int)list.size(),
+                                     b->buf[(b->cur + k) & 1],
                                      b->buf[(b->cur + k + 1) & 1], b->d_step_scratch, 0, nullptr, b->stream);
             }
             if (e == hipSuccess) e = hipEventRecord(b->ev1, b->stream);

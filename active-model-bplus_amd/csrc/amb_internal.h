@@ -16,7 +16,7 @@ void clear_error();
 int check_device();
 int validate_params(int64_t ny, int64_t nx, const amb_stencil* st, const amb_model* m);
 int grid_tiles(int64_t n, int t);
-int default_tile(int precision, int64_t ny, int64_t nx);
+int default_tile(int precision, int64_t ny, int64_t nx, int noise);
 int xcd_chunk(int nblocks);
 
 template <typename T>

@@ -47,16 +47,16 @@ struct TileShape {
 
 // Compiled tile configurations (index = autotuner choice): X(index, TW, TH, NT).  The
 // default tile follows an order of preference per kernel (default_tile in amb_capi.hip).
-#define AMB_F32_TILES(X) X(0, 32, 48, 512) X(1, 32, 24, 256) X(2, 32, 32, 256) X(3, 64, 32, 512) X(4, 64, 16, 256) X(5, 32, 16, 256) X(6, 16, 16, 128)
-#define AMB_F64_TILES(X) X(0, 32, 32, 256) X(1, 64, 16, 256) X(2, 32, 16, 256) X(3, 32, 32, 512) X(4, 16, 16, 128)
+#define AMB_F32_TILES(X) X(0, 32, 48, 512) X(1, 32, 24, 256) X(2, 32, 32, 256) X(3, 64, 32, 512) X(4, 64, 16, 256) X(5, 32, 16, 256) X(6, 16, 16, 128) X(7, 16, 8, 128) X(8, 8, 8, 64)
+#define AMB_F64_TILES(X) X(0, 32, 32, 256) X(1, 64, 16, 256) X(2, 32, 16, 256) X(3, 32, 32, 512) X(4, 16, 16, 128) X(5, 16, 8, 128) X(6, 8, 8, 64)
 // FP32 orders of preference, measured on MI355X.  Noisy: 32x48 is fastest at 4096^2
 // (113.8 us per step, 116-139 us for the others) and 64x32 at 1024^2, where a single round
 // of 64x32 workgroups pays least halo (11.0 us, 32x32 11.6 us).  Noiseless: the kernel holds
 // its 17-tap windows in registers (92 VGPRs instead of 73), so 256-thread workgroups pack
 // better; 32x24 is fastest at 4096^2 (98.0 us, 32x48 110.4 us) and 64x32 at 1024^2 (9.3 us).
 // 16x16 serves the smallest fields (256^2: 5.3 us noiseless, 32x16 5.6 us).
-#define AMB_F32_NOISY_ORDER 0, 3, 2, 1, 4, 5, 6
-#define AMB_F32_NOISELESS_ORDER 1, 3, 2, 5, 0, 4, 6
+#define AMB_F32_NOISY_ORDER 0, 3, 2, 1, 4, 5, 6, 7, 8
+#define AMB_F32_NOISELESS_ORDER 1, 3, 2, 5, 0, 4, 6, 7, 8
 
 // Tile shapes compiled per precision (index = autotuner choice).
 int tile_count(int precision);

@@ -23,6 +23,8 @@
 
 #include <stdint.h>
 
+#include <cmath>
+
 namespace amb {
 
 // D8 taps for offsets 1..4 (antisymmetric).
@@ -84,6 +86,100 @@ __host__ __device__ inline U4 noise_counter(uint32_t group, uint32_t row, uint64
     c.z = (uint32_t)step;
     c.w = ((uint32_t)(step >> 32) & 0x7fffffffu) | ((uint32_t)comp << 31);
     return c;
+}
+
+// Mantissa in [1/2, 1) and exponent of a finite non-zero double (two instructions on gfx950).
+__host__ __device__ inline double frexp_mant(double u, int* e)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    *e = __builtin_amdgcn_frexp_exp(u);
+    return __builtin_amdgcn_frexp_mant(u);
+#else
+    return std::frexp(u, e);
+#endif
+}
+
+// FP64 Box-Muller on the two uniforms a Philox word pair gives.  The generic math library
+// handles every input (zeros, infinities, huge arguments, denormals) with special-case code
+// that dominated the FP64 noisy step; here the inputs are known, so two short kernels of
+// our own suffice, both accurate to a few ulp:
+//   ln u for u = (w + 1/2) 2^-32 in [2^-33, 1): u = m 2^e with m in [sqrt(1/2), sqrt(2)),
+//     ln m = 2 atanh(s), s = f / (2 + f), f = m - 1 (exact), |s| <= 0.1716, Taylor series
+//     of atanh to s^21 (truncation < 1e-18 relative); e ln 2 with ln 2 split in a high
+//     part whose products with |e| <= 34 are exact and a low part;
+//   sin and cos of 2 pi u for u = w 2^-32 in [0, 1): quadrant j = rint(4u), reduced
+//     argument x = (4u - j) pi/2 in [-pi/4, pi/4] (4u - j is exact), Taylor series of sin
+//     to x^17 and of cos to x^16 (truncation < 1e-16 relative), then the quadrant's
+//     swap and signs.
+__host__ __device__ inline double ln_unit(double u)
+{
+    int e;
+    double m = frexp_mant(u, &e);  // u = m 2^e, m in [1/2, 1)
+    if (m < 0.70710678118654752440) {
+        m *= 2.0;
+        e -= 1;
+    }
+    const double f = m - 1.0;
+    const double s = f / (2.0 + f);
+    const double s2 = s * s;
+    double p = 1.0 / 21.0;
+    p = fma(p, s2, 1.0 / 19.0);
+    p = fma(p, s2, 1.0 / 17.0);
+    p = fma(p, s2, 1.0 / 15.0);
+    p = fma(p, s2, 1.0 / 13.0);
+    p = fma(p, s2, 1.0 / 11.0);
+    p = fma(p, s2, 1.0 / 9.0);
+    p = fma(p, s2, 1.0 / 7.0);
+    p = fma(p, s2, 1.0 / 5.0);
+    p = fma(p, s2, 1.0 / 3.0);
+    const double ts = 2.0 * s;
+    const double lnm = fma(ts * s2, p, ts);  // 2 s (1 + s^2 p)
+    const double de = (double)e;
+    constexpr double kLn2Hi = 6.93147180369123816490e-01;  // 32 significant bits: de * kLn2Hi is exact
+    constexpr double kLn2Lo = 1.90821492927058770002e-10;
+    return fma(de, kLn2Hi, fma(de, kLn2Lo, lnm));
+}
+
+__host__ __device__ inline void sincos_turn(double u, double& s, double& c)
+{
+    const double q = 4.0 * u;
+    const double j = rint(q);
+    const double x = (q - j) * 1.57079632679489661923;  // (4u - j) pi/2
+    const double x2 = x * x;
+    // sin x = x + x^3 S(x^2), cos x = 1 - x^2/2 + x^4 C(x^2)
+    double sp = 1.0 / 355687428096000.0;                // 1/17!
+    sp = fma(sp, x2, -1.0 / 1307674368000.0);           // -1/15!
+    sp = fma(sp, x2, 1.0 / 6227020800.0);               // 1/13!
+    sp = fma(sp, x2, -1.0 / 39916800.0);                // -1/11!
+    sp = fma(sp, x2, 1.0 / 362880.0);                   // 1/9!
+    sp = fma(sp, x2, -1.0 / 5040.0);                    // -1/7!
+    sp = fma(sp, x2, 1.0 / 120.0);                      // 1/5!
+    sp = fma(sp, x2, -1.0 / 6.0);                       // -1/3!
+    double cp = 1.0 / 20922789888000.0;                 // 1/16!
+    cp = fma(cp, x2, -1.0 / 87178291200.0);             // -1/14!
+    cp = fma(cp, x2, 1.0 / 479001600.0);                // 1/12!
+    cp = fma(cp, x2, -1.0 / 3628800.0);                 // -1/10!
+    cp = fma(cp, x2, 1.0 / 40320.0);                    // 1/8!
+    cp = fma(cp, x2, -1.0 / 720.0);                     // -1/6!
+    cp = fma(cp, x2, 1.0 / 24.0);                       // 1/4!
+    const double sx = fma(x * x2, sp, x);
+    const double cx = fma(x2 * x2, cp, fma(-0.5, x2, 1.0));
+    const int k = (int)j & 3;
+    const double a = (k & 1) ? cx : sx;  // sin(theta) up to sign
+    const double b = (k & 1) ? sx : cx;  // cos(theta) up to sign
+    s = (k & 2) ? -a : a;
+    c = (k == 1 || k == 2) ? -b : b;
+}
+
+__host__ __device__ inline void box_muller(uint32_t wa, uint32_t wb, double& z0, double& z1)
+{
+    const double u1 = ((double)wa + 0.5) * 0x1p-32;
+    const double u2 = (double)wb * 0x1p-32;
+    const double r = sqrt(-2.0 * ln_unit(u1));
+    double s, c;
+    sincos_turn(u2, s, c);  // sin and cos of 2 pi u2
+    z0 = r * c;
+    z1 = r * s;
 }
 
 }  // namespace amb

@@ -58,17 +58,6 @@ __device__ __forceinline__ void box_muller(uint32_t wa, uint32_t wb, float& z0, 
     z1 = r * __builtin_amdgcn_sinf(u2);
 }
 
-__device__ __forceinline__ void box_muller(uint32_t wa, uint32_t wb, double& z0, double& z1)
-{
-    const double u1 = ((double)wa + 0.5) * 0x1p-32;
-    const double u2 = (double)wb * 0x1p-32;
-    const double r = sqrt(-2.0 * log(u1));
-    double s, c;
-    sincospi(2.0 * u2, &s, &c);  // sin and cos of 2 pi u2 (2 u2 is exact; no range reduction by pi)
-    z0 = r * c;
-    z1 = r * s;
-}
-
 template <typename T>
 __device__ __forceinline__ void normals4(uint32_t group, uint32_t row, uint64_t step, int comp,
                                          uint32_t k0, uint32_t k1, T z[4])

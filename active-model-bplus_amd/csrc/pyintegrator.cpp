@@ -482,6 +482,15 @@ class Ensemble {
         check(amb_batch_set_tiles(b_, t.data()));
     }
 
+    void set_resident(bool on) { check(amb_batch_set_resident(b_, on ? 1 : 0)); }
+
+    int64_t resident_count() const
+    {
+        int64_t n = 0;
+        check(amb_batch_resident_count(b_, &n));
+        return n;
+    }
+
     std::vector<int> autotune(int reps)
     {
         std::vector<int> t(shapes_.size());
@@ -626,6 +635,9 @@ PYBIND11_MODULE(integrator, m)
              "(FP64 sum over all realisations, number of non-finite sites) found by the last run")
         .def_property_readonly("tiles", &Ensemble::tiles)
         .def("set_tiles", &Ensemble::set_tiles, py::arg("tiles"))
+        .def("set_resident", &Ensemble::set_resident, py::arg("on"),
+             "keep members that fit in LDS resident across steps (default on)")
+        .def_property_readonly("resident_count", &Ensemble::resident_count)
         .def("autotune", &Ensemble::autotune, py::arg("reps") = 4)
         .def_property_readonly("timestep", &Ensemble::timestep);
 

@@ -15,7 +15,7 @@ times the CPU FP64 oracle (oracle/, the checker) on this host on a bounded sampl
 
 Other workloads (--workload): c1 (256^2 FP64 spinodal), c2 (1024^2 FP32 noisy
 active), c3 (default), c3t0 (the C3 model without noise: the deterministic path), c3f64
-(4096^2 FP64), c3x8192 (the C3 model on an 8192^2 FP32
+(4096^2 FP64), c3f64t0 (the same without noise), c3x8192 (the C3 model on an 8192^2 FP32
 field: phi^n and phi^(n+1) together are 512 MiB, past the 256 MB MALL), c4 (batched ensemble of 512
 realisations, sides 64..512, vs sequential plain runs on the <=256^2 subset), c5
 (learned surrogate with Euler audits vs plain Euler, 256^2; --steps = coarse steps of
@@ -61,6 +61,8 @@ WORKLOADS = {
                  init=("normal", 0.0, 0.05), label="C3 model at T = 0 (noiseless) 4096x4096 FP32"),
     "c3f64": dict(n=4096, precision=64, seed=3, stencil=(5e-3, 1, 1), model=(-0.25, 0, 0.25, 1, 0.5, -1, 0.05),
                   init=("normal", 0.0, 0.05), label="C3 AMB+ noisy 4096x4096 FP64"),
+    "c3f64t0": dict(n=4096, precision=64, seed=3, stencil=(5e-3, 1, 1), model=(-0.25, 0, 0.25, 1, 0.5, -1, 0.0),
+                    init=("normal", 0.0, 0.05), label="C3 model at T = 0 (noiseless) 4096x4096 FP64"),
     "c3x8192": dict(n=8192, precision=32, seed=3, stencil=(5e-3, 1, 1), model=(-0.25, 0, 0.25, 1, 0.5, -1, 0.05),
                     init=("normal", 0.0, 0.05), label="C3 AMB+ noisy 8192x8192 FP32 (beyond the MALL)"),
 }
@@ -233,7 +235,16 @@ def bench_batched(args):
         return total
 
     steps = args.steps
-    t_all, _ = batched(list(range(count)), steps)
+    t_all, ens = batched(list(range(count)), steps)
+    if args.batched_only:
+        # one ensemble, nothing else (a profiler run of the batched kernels alone)
+        sites_all = int(sum(s * s for s in sides))
+        return {"metric": METRIC, "value": sites_all * steps / t_all, "unit": "site-updates/s", "n_gpus": 1,
+                "steps": steps, "warmup": args.warmup, "ms_per_step": t_all * 1e3 / steps, "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded numpy)",
+                "config": {"workload": f"batched ensemble of {count} realisations (batched leg only)",
+                           "sites": sites_all, "resident_members": ens.resident_count}}
+    del ens
     t_small, _ = batched(small, steps)
     seq_steps = max(32, steps // 10)
     t_seq_small = sequential(small, seq_steps) * steps / seq_steps
@@ -362,6 +373,7 @@ def main():
     p.add_argument("--audit-every", type=int, default=10, help="c5: initial coarse steps per Euler audit")
     p.add_argument("--max-audit-every", type=int, default=200, help="c5: longest audit interval")
     p.add_argument("--realisations", type=int, default=512)
+    p.add_argument("--batched-only", action="store_true", help="c4: time the batched ensemble only")
     p.add_argument("--copy-side", type=int, default=4096, help="copy: side of the square FP32 array")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     p.add_argument("--autotune", action="store_true", help="pick the tile shape by timing")

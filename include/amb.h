@@ -173,6 +173,15 @@ int amb_batch_last_mass(amb_batch* b, double* sum, int64_t* nonfinite);
 int amb_batch_set_tiles(amb_batch* b, const int* tile_per_realisation);
 int amb_batch_get_tiles(amb_batch* b, int* tile_per_realisation);
 
+/* Resident members: with enable != 0 (the default) every realisation whose two field
+ * buffers and one tile's scratch fit in 80 KiB of LDS (FP32 up to 64x64, FP64 up to about
+ * 44x44 sites) is advanced by one workgroup that keeps it in LDS: one launch per run
+ * chunk advances all of them by many steps, on a graph branch beside the tiled members'
+ * per-step launches, with no inter-workgroup synchronisation.  Results are the same bits
+ * either way.  count receives the number of resident members. */
+int amb_batch_set_resident(amb_batch* b, int enable);
+int amb_batch_resident_count(amb_batch* b, int64_t* count);
+
 /* Autotuner: for every distinct grid size, times `reps` steps of every compiled
  * tile shape and keeps the fastest (state and step counter are restored).
  * `chosen` (nullable) receives the tile index per realisation. */
