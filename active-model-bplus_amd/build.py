@@ -23,7 +23,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIBAMB = os.path.join(LIBDIR, "libamb.so")
 PKG = os.path.join(HERE, "activemodelbplus", "integrator")
 
-HIP_SOURCES = ["amb_step.hip", "amb_capi.hip", "amb_batch.hip", "amb_unet.hip", "amb_sk.hip"]
+HIP_SOURCES = ["amb_step.hip", "amb_capi.hip", "amb_batch.hip", "amb_unet.hip", "amb_sk.hip", "amb_noalloc.hip"]
 ARCH = os.environ.get("AMB_OFFLOAD_ARCH", "gfx950")
 
 # Packed FP32 VALU ops are disabled for all device code.

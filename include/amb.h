@@ -143,6 +143,26 @@ const char* amb_last_error(void);
 /* Number of visible HIP devices (0 without a GPU; never fails). */
 int amb_device_count(int* count);
 
+/* ---- allocation-free run ---------------------------------------------------- */
+
+/* amb_run's steps on caller-owned device memory, allocating nothing: no counterpart in
+ * the reference; it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  `field` and
+ * `work` are two distinct device buffers of ny*nx elements of `precision` (float for
+ * AMB_F32, double for AMB_F64); the field advances nsteps explicit-Euler steps with the
+ * noise counter starting at `timestep` and key `seed` (exactly amb_run's results for a
+ * handle with the same options) and ends in `field`; `work` is overwritten.  `stream`
+ * is a hipStream_t (NULL: the default stream); the call returns after synchronising it.
+ * mass / nonfinite (nullable) receive the FP64 sum and the non-finite count of the new
+ * field; AMB_E_NONFINITE is returned when that count is not zero.  The library's
+ * scratch (step counter and the fused check's per-tile partials) is one static device
+ * array of 1 MiB + 256 B per GPU, shared by every call on that GPU: fields with more
+ * than 65536 tiles of the default tile shape are rejected (AMB_E_ARG), and calls that
+ * overlap on one GPU race. */
+int amb_run_noalloc(void* field, void* work, int64_t ny, int64_t nx, int precision, const amb_stencil* stencil,
+                    const amb_model* model, uint64_t seed, int64_t timestep, int64_t nsteps, void* stream,
+                    double* mass, int64_t* nonfinite);
+
 /* ---- batched ensemble (no reference counterpart) ------------------------- */
 
 /* One independent realisation: its own grid, stencil, model and noise seed. */
