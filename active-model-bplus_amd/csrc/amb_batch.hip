@@ -314,8 +314,10 @@ struct amb_batch {
     int64_t resident_quiet = 0;     // how many of them are noiseless
     int* d_resident = nullptr;
     int64_t resident_max_n = 0;
-    hipStream_t side = nullptr;     // second capture branch: the resident launch runs beside the tiled steps
+    hipStream_t side = nullptr;     // capture branch of the resident launch, beside the tiled steps
     hipEvent_t fork = nullptr, join = nullptr;
+    std::vector<hipStream_t> branch;      // capture branch per tile-shape group
+    std::vector<hipEvent_t> branch_done;
     double* d_part = nullptr;
     unsigned long long* d_bad_part = nullptr;
     double* d_sum = nullptr;
@@ -353,6 +355,9 @@ void release(amb_batch* b)
                     (void*)b->d_bad_part, (void*)b->d_sum, (void*)b->d_bad, (void*)b->d_tile_part,
                     (void*)b->d_resident})
         if (p) (void)hipFree(p);
+    for (hipStream_t st : b->branch) (void)hipStreamSynchronize(st);
+    for (hipEvent_t ev : b->branch_done) (void)hipEventDestroy(ev);
+    for (hipStream_t st : b->branch) (void)hipStreamDestroy(st);
     if (b->fork) (void)hipEventDestroy(b->fork);
     if (b->join) (void)hipEventDestroy(b->join);
     if (b->side) (void)hipStreamDestroy(b->side);
@@ -511,19 +516,44 @@ hipError_t launch_resident_members(amb_batch* b, int parity_in, int nsteps, hipS
 // its own (fork from and join to the main stream), beside the tiled members' steps.
 int build_graph(amb_batch* b, int parity)
 {
+    // one capture branch per tile-shape group (its kBatchGraphSteps steps in a chain) and
+    // one for the resident members: the groups touch disjoint members, so their chains run
+    // side by side and one group's last, partial round of workgroups overlaps the others'
+    while (b->branch.size() < b->groups.size()) {
+        hipStream_t st = nullptr;
+        hipEvent_t ev = nullptr;
+        hipError_t ce = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        if (ce == hipSuccess) ce = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (ce != hipSuccess) {
+            if (st) (void)hipStreamDestroy(st);
+            return fail_hip(ce, "branch stream");
+        }
+        b->branch.push_back(st);
+        b->branch_done.push_back(ev);
+    }
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return fail_hip(e, "hipStreamBeginCapture");
-    const bool branch = !b->resident.empty();
-    if (branch) {
-        e = hipEventRecord(b->fork, b->stream);
-        if (e == hipSuccess) e = hipStreamWaitEvent(b->side, b->fork, 0);
+    e = hipEventRecord(b->fork, b->stream);
+    if (e == hipSuccess && !b->resident.empty()) {
+        e = hipStreamWaitEvent(b->side, b->fork, 0);
         if (e == hipSuccess) e = launch_resident_members(b, parity, kBatchGraphSteps, b->side);
         if (e == hipSuccess) e = hipEventRecord(b->join, b->side);
     }
-    for (int k = 0; k < kBatchGraphSteps && e == hipSuccess; ++k)
-        e = launch_groups(b, parity + k, b->d_step, k, k == kBatchGraphSteps - 1);
-    if (e == hipSuccess && branch) e = hipStreamWaitEvent(b->stream, b->join, 0);
+    for (size_t i = 0; i < b->groups.size() && e == hipSuccess; ++i) {
+        const Group& gr = b->groups[i];
+        hipStream_t st = b->branch[i];
+        e = hipStreamWaitEvent(st, b->fork, 0);
+        for (int k = 0; k < kBatchGraphSteps && e == hipSuccess; ++k) {
+            const int pin = parity + k;
+            e = launch_batch(b->precision, gr.tile, gr.noise, b->d_desc, gr.d_tiles, gr.ntiles, b->buf[pin & 1],
+                             b->buf[(pin + 1) & 1], b->d_step, k,
+                             k == kBatchGraphSteps - 1 ? b->d_tile_part + 2 * gr.part_base : nullptr, st);
+        }
+        if (e == hipSuccess) e = hipEventRecord(b->branch_done[i], st);
+    }
+    for (size_t i = 0; i < b->groups.size() && e == hipSuccess; ++i) e = hipStreamWaitEvent(b->stream, b->branch_done[i], 0);
+    if (e == hipSuccess && !b->resident.empty()) e = hipStreamWaitEvent(b->stream, b->join, 0);
     if (e == hipSuccess) e = launch_advance(b->d_step, kBatchGraphSteps, b->stream);
     const hipError_t e2 = hipStreamEndCapture(b->stream, &g);
     if (e != hipSuccess) {
