@@ -119,8 +119,8 @@ int validate_params(int64_t ny, int64_t nx, const amb_stencil* st, const amb_mod
 
 int grid_tiles(int64_t n, int t) { return (int)((n + t - 1) / t); }
 
-// Default tile: in order of preference (AMB_F32_NOISY_ORDER, AMB_F32_NOISELESS_ORDER, or
-// the FP64 table order), the first shape whose tiles fill the device's resident workgroups
+// Default tile: in order of preference (AMB_F32_NOISY_ORDER, AMB_F32_NOISELESS_ORDER or
+// AMB_F64_ORDER), the first shape whose tiles fill the device's resident workgroups
 // either for at least 8 rounds or in whole-or-mostly-full rounds (a last round under half
 // full leaves most CUs idle for a round).  Fields too small for one round of any shape take
 // the shape whose tile count is nearest two per CU (in ratio; the larger count on a tie):
@@ -136,13 +136,16 @@ int default_tile(int precision, int64_t ny, int64_t nx, int noise)
     }
     static constexpr int kNoisyF32[] = {AMB_F32_NOISY_ORDER};
     static constexpr int kNoiselessF32[] = {AMB_F32_NOISELESS_ORDER};
+    static constexpr int kF64[] = {AMB_F64_ORDER};
 #define AMB_COUNT_ONE(i, tw, th, nt) +1
     static_assert(sizeof(kNoisyF32) / sizeof(kNoisyF32[0]) == 0 AMB_F32_TILES(AMB_COUNT_ONE) &&
                       sizeof(kNoiselessF32) / sizeof(kNoiselessF32[0]) == 0 AMB_F32_TILES(AMB_COUNT_ONE),
                   "the FP32 orders must list every FP32 tile shape");
+    static_assert(sizeof(kF64) / sizeof(kF64[0]) == 0 AMB_F64_TILES(AMB_COUNT_ONE),
+                  "the FP64 order must list every FP64 tile shape");
 #undef AMB_COUNT_ONE
     const int n = tile_count(precision);
-    const int* order = precision != AMB_F32 ? nullptr : noise == kNoiseNone ? kNoiselessF32 : kNoisyF32;
+    const int* order = precision != AMB_F32 ? kF64 : noise == kNoiseNone ? kNoiselessF32 : kNoisyF32;
     int best = 0;
     double best_gap = 0.0;
     int64_t best_tiles = 0;

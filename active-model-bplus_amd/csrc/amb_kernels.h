@@ -57,6 +57,11 @@ struct TileShape {
 // 16x16 serves the smallest fields (256^2: 5.3 us noiseless, 32x16 5.6 us).
 #define AMB_F32_NOISY_ORDER 0, 3, 2, 1, 4, 5, 6, 7, 8
 #define AMB_F32_NOISELESS_ORDER 1, 3, 2, 5, 0, 4, 6, 7, 8
+// FP64, noisy and noiseless alike: 32x32 tiles of 512 threads (held to 128 VGPRs, so two
+// workgroups, 16 waves, share a CU) are fastest at 4096^2 (294.5 / 182.8 us per step against
+// 324.1 / 208.7 us for 256 threads); 16x8 serves the smallest fields (256^2: 5.64 us, 16x16
+// 5.79 us, 8x8 5.92 us).
+#define AMB_F64_ORDER 3, 0, 2, 1, 4, 5, 6
 
 // Tile shapes compiled per precision (index = autotuner choice).
 int tile_count(int precision);

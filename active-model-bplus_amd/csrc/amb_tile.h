@@ -11,6 +11,15 @@ namespace amb {
 
 constexpr int kThreads = 256;
 
+// Waves per SIMD the register allocation must leave room for: FP64 workgroups of 512
+// threads are held to 128 VGPRs (4 waves per SIMD), so two of them (16 waves) share a CU
+// as their LDS allows; every other kernel is left to the compiler (1 = no constraint).
+template <typename T, int NT>
+constexpr int min_waves_per_eu()
+{
+    return (sizeof(T) == 8 && NT >= 512) ? 4 : 1;
+}
+
 __device__ __forceinline__ int wrap_index(int g, int n)
 {
     // Exact for any g; one iteration in the common case, several on tiny grids.
