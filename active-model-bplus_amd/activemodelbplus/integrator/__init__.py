@@ -24,9 +24,11 @@ from .integrator import CudaException, HipException, Model, Stencil
 from .integrator import Ensemble as _BaseEnsemble
 from .integrator import Integrator as BaseIntegrator
 from .integrator import Surrogate, device_count, tile_shapes
+from .integrator import tuning_clear, tuning_entries, tuning_get, tuning_set
 
 __all__ = ["Model", "Stencil", "Integrator", "Ensemble", "Surrogate", "CudaException", "HipException",
-           "steps_from_time", "step_iterator", "plot_field", "device_count", "tile_shapes"]
+           "steps_from_time", "step_iterator", "plot_field", "device_count", "tile_shapes",
+           "tuning_set", "tuning_get", "tuning_entries", "tuning_clear"]
 
 
 def plot_field(field, fig=None, axes=None, fontsize=8):

@@ -670,6 +670,9 @@ int amb_batch_create(const amb_realisation* desc, int64_t count, const double* c
         o += desc[r].ny * desc[r].nx;
     }
     b->tile_of.assign(count, 0);
+    for (int64_t r = 0; r < count; ++r)  // the tuning table's entries where it has them
+        (void)tuned_tile(precision, desc[r].ny, desc[r].nx, desc[r].model.T > 0.0 ? kNoisePhilox : kNoiseNone,
+                         &b->tile_of[r]);
     hipError_t e;
 #define AMB_CHECK(call)                           \
     do {                                          \
@@ -858,6 +861,8 @@ int amb_batch_autotune(amb_batch* b, int reps, int* chosen)
 {
     clear_error();
     if (!b) return fail(AMB_E_ARG, "batch must not be null");
+    const int reps_in = reps;
+    if (reps < 0) reps = -reps;
     if (reps < 1) reps = 4;
     hipError_t e = hipStreamSynchronize(b->stream);
     if (e != hipSuccess) return fail_hip(e, "hipStreamSynchronize");
@@ -876,8 +881,15 @@ int amb_batch_autotune(amb_batch* b, int reps, int* chosen)
     if (e == hipSuccess) e = hipMemcpy(b->d_step_scratch, &zero, sizeof(long long), hipMemcpyHostToDevice);
     const int nt = tile_count(b->precision);
     std::vector<int> best_tile = b->tile_of;  // resident members keep theirs (unused while resident)
+    const bool prefer_table = reps_in < 0;
     for (auto& kv : classes) {
         if (e != hipSuccess) break;
+        int known = 0;
+        if (prefer_table && tuned_tile(b->precision, std::get<0>(kv.first), std::get<1>(kv.first),
+                                       std::get<2>(kv.first), &known)) {
+            for (int r : kv.second) best_tile[r] = known;
+            continue;
+        }
         float best = 1e30f;
         int best_t = 0;
         for (int t = 0; t < nt && e == hipSuccess; ++t) {
@@ -915,6 +927,8 @@ int amb_batch_autotune(amb_batch* b, int reps, int* chosen)
             }
         }
         for (int r : kv.second) best_tile[r] = best_t;
+        if (e == hipSuccess)
+            record_tile(b->precision, std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first), best_t);
     }
     if (e == hipSuccess) e = hipMemcpy(b->buf[b->cur], saved, (size_t)b->total * b->esize, hipMemcpyDeviceToDevice);
     (void)hipFree(saved);

@@ -12,8 +12,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "amb.h"
@@ -168,6 +171,36 @@ int default_tile(int precision, int64_t ny, int64_t nx, int noise)
 
 // XCD-contiguous tile order: gfx950 dispatches workgroup i to XCD i % 8.
 int xcd_chunk(int nblocks) { return (nblocks % 8 == 0 && nblocks >= 64) ? nblocks / 8 : 0; }
+
+// ---- tuning table: the autotuners' choices, per (precision, grid, noise) ------------
+namespace {
+std::mutex g_tune_mutex;
+std::map<std::tuple<int, int64_t, int64_t, int>, int> g_tune;
+}  // namespace
+
+bool tuned_tile(int precision, int64_t ny, int64_t nx, int noise, int* tile)
+{
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    const auto it = g_tune.find({precision, ny, nx, noise != kNoiseNone});
+    if (it == g_tune.end()) return false;
+    *tile = it->second;
+    return true;
+}
+
+void record_tile(int precision, int64_t ny, int64_t nx, int noise, int tile)
+{
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    g_tune[{precision, ny, nx, noise != kNoiseNone}] = tile;
+}
+
+// The tile a new integrator starts with: the tuning table's entry when there is one,
+// otherwise the occupancy rule.
+int initial_tile(int precision, int64_t ny, int64_t nx, int noise)
+{
+    int t = 0;
+    if (tuned_tile(precision, ny, nx, noise, &t)) return t;
+    return default_tile(precision, ny, nx, noise);
+}
 
 }  // namespace amb
 
@@ -562,7 +595,7 @@ int amb_create(const double* field, int64_t ny, int64_t nx, const amb_stencil* s
     AMB_CHECK(hipGetDevice(&h->device));
     AMB_CHECK(prepare_step_kernels());
     h->tile = opt.tile > 0 ? opt.tile - 1
-                           : default_tile(opt.precision, ny, nx, model->T > 0.0 ? kNoisePhilox : kNoiseNone);
+                           : initial_tile(opt.precision, ny, nx, model->T > 0.0 ? kNoisePhilox : kNoiseNone);
     AMB_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     AMB_CHECK(hipMalloc(&h->buf[0], n * h->esize));
     AMB_CHECK(hipMalloc(&h->buf[1], n * h->esize));
@@ -745,6 +778,16 @@ int amb_autotune(amb_handle* h, int reps, int* chosen)
 {
     clear_error();
     if (!h) return fail(AMB_E_ARG, "handle must not be null");
+    const int mode0 = h->model.T > 0.0 ? kNoisePhilox : kNoiseNone;
+    if (reps < 0) {  // the tuning table's choice, when it has one
+        int t = 0;
+        if (tuned_tile(h->precision, h->ny, h->nx, mode0, &t)) {
+            h->tile = t;
+            if (chosen) *chosen = t;
+            return AMB_OK;
+        }
+        reps = -reps;
+    }
     if (reps < 1) reps = 8;
     const size_t bytes = (size_t)(h->ny * h->nx) * h->esize;
     amb_handle trial = {};
@@ -791,7 +834,55 @@ int amb_autotune(amb_handle* h, int reps, int* chosen)
     trial.stream = nullptr;
     if (e != hipSuccess) return fail_hip(e, "autotune");
     h->tile = best_t;
+    record_tile(h->precision, h->ny, h->nx, mode0, best_t);
     if (chosen) *chosen = best_t;
+    return AMB_OK;
+}
+
+int amb_tuning_set(int precision, int64_t ny, int64_t nx, int noisy, int tile)
+{
+    clear_error();
+    if (precision != AMB_F32 && precision != AMB_F64) return fail(AMB_E_ARG, "bad precision");
+    if (ny <= 0 || nx <= 0) return fail(AMB_E_ARG, "bad grid");
+    if (tile < 0 || tile >= tile_count(precision)) return fail(AMB_E_ARG, "tile index out of range");
+    record_tile(precision, ny, nx, noisy ? kNoisePhilox : kNoiseNone, tile);
+    return AMB_OK;
+}
+
+int amb_tuning_get(int precision, int64_t ny, int64_t nx, int noisy, int* tile)
+{
+    clear_error();
+    if (!tile) return fail(AMB_E_ARG, "tile must not be null");
+    if (!tuned_tile(precision, ny, nx, noisy ? kNoisePhilox : kNoiseNone, tile)) *tile = -1;
+    return AMB_OK;
+}
+
+int amb_tuning_count(int64_t* count)
+{
+    if (!count) return fail(AMB_E_ARG, "count must not be null");
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    *count = (int64_t)g_tune.size();
+    return AMB_OK;
+}
+
+int amb_tuning_entry(int64_t index, int* precision, int64_t* ny, int64_t* nx, int* noisy, int* tile)
+{
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    if (index < 0 || index >= (int64_t)g_tune.size()) return fail(AMB_E_ARG, "tuning entry out of range");
+    auto it = g_tune.begin();
+    std::advance(it, index);
+    if (precision) *precision = std::get<0>(it->first);
+    if (ny) *ny = std::get<1>(it->first);
+    if (nx) *nx = std::get<2>(it->first);
+    if (noisy) *noisy = std::get<3>(it->first);
+    if (tile) *tile = it->second;
+    return AMB_OK;
+}
+
+int amb_tuning_clear(void)
+{
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    g_tune.clear();
     return AMB_OK;
 }
 

@@ -17,6 +17,10 @@ int check_device();
 int validate_params(int64_t ny, int64_t nx, const amb_stencil* st, const amb_model* m);
 int grid_tiles(int64_t n, int t);
 int default_tile(int precision, int64_t ny, int64_t nx, int noise);
+// The autotuners' table (process-wide): lookup, record, and the tile a new field starts with.
+bool tuned_tile(int precision, int64_t ny, int64_t nx, int noise, int* tile);
+void record_tile(int precision, int64_t ny, int64_t nx, int noise, int tile);
+int initial_tile(int precision, int64_t ny, int64_t nx, int noise);
 int xcd_chunk(int nblocks);
 
 template <typename T>

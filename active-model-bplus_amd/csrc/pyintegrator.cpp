@@ -230,12 +230,12 @@ class Integrator {
 
     amb_handle* handle() const { return h_; }
 
-    int autotune(int reps)
+    int autotune(int reps, bool cached)
     {
         int best = 0, rc;
         {
             py::gil_scoped_release release;
-            rc = amb_autotune(h_, reps, &best);
+            rc = amb_autotune(h_, cached ? -reps : reps, &best);
         }
         check(rc);
         return best;
@@ -491,13 +491,13 @@ class Ensemble {
         return n;
     }
 
-    std::vector<int> autotune(int reps)
+    std::vector<int> autotune(int reps, bool cached)
     {
         std::vector<int> t(shapes_.size());
         int rc;
         {
             py::gil_scoped_release release;
-            rc = amb_batch_autotune(b_, reps, t.data());
+            rc = amb_batch_autotune(b_, cached ? -reps : reps, t.data());
         }
         check(rc);
         return t;
@@ -597,8 +597,9 @@ PYBIND11_MODULE(integrator, m)
         .def("set_noise", &Integrator::set_noise, py::arg("noise"),
              "test only: explicit unscaled normals (steps, 2, ny, nx) for the next steps")
         .def("set_tile", &Integrator::set_tile, py::arg("index"))
-        .def("autotune", &Integrator::autotune, py::arg("reps") = 8,
-             "time every compiled tile shape on a copy of the state and keep the fastest")
+        .def("autotune", &Integrator::autotune, py::arg("reps") = 8, py::arg("cached") = false,
+             "time every compiled tile shape on a copy of the state and keep the fastest "
+             "(cached: take the tuning table's entry when there is one)")
         .def("set_field", &Integrator::set_field, py::arg("field"), "replace the field (step counter unchanged)")
         .def("structure_factor_bins", &Integrator::structure_factor_bins, py::arg("edges"),
              "(sums, counts) of |phi_k|^2 per numpy.digitize(|k|, edges, right=True) index, on the device");
@@ -638,7 +639,8 @@ PYBIND11_MODULE(integrator, m)
         .def("set_resident", &Ensemble::set_resident, py::arg("on"),
              "keep members that fit in LDS resident across steps (default on)")
         .def_property_readonly("resident_count", &Ensemble::resident_count)
-        .def("autotune", &Ensemble::autotune, py::arg("reps") = 4)
+        .def("autotune", &Ensemble::autotune, py::arg("reps") = 4, py::arg("cached") = false,
+             "time every tile shape per grid size and noise mode (cached: take the tuning table's entries first)")
         .def_property_readonly("timestep", &Ensemble::timestep);
 
     m.def("device_count", []() {
@@ -646,6 +648,27 @@ PYBIND11_MODULE(integrator, m)
         amb_device_count(&n);
         return n;
     });
+    m.def("tuning_set", [](int precision, int64_t ny, int64_t nx, bool noisy, int tile) {
+        check(amb_tuning_set(precision, ny, nx, noisy ? 1 : 0, tile));
+    }, py::arg("precision"), py::arg("ny"), py::arg("nx"), py::arg("noisy"), py::arg("tile"));
+    m.def("tuning_get", [](int precision, int64_t ny, int64_t nx, bool noisy) {
+        int t = -1;
+        check(amb_tuning_get(precision, ny, nx, noisy ? 1 : 0, &t));
+        return t;
+    }, py::arg("precision"), py::arg("ny"), py::arg("nx"), py::arg("noisy"));
+    m.def("tuning_entries", []() {
+        int64_t n = 0;
+        check(amb_tuning_count(&n));
+        std::vector<std::tuple<int, int64_t, int64_t, bool, int>> out;
+        for (int64_t i = 0; i < n; ++i) {
+            int p = 0, noisy = 0, t = 0;
+            int64_t ny = 0, nx = 0;
+            check(amb_tuning_entry(i, &p, &ny, &nx, &noisy, &t));
+            out.emplace_back(p, ny, nx, noisy != 0, t);
+        }
+        return out;
+    }, "(precision, ny, nx, noisy, tile index) per entry of the tuning table");
+    m.def("tuning_clear", []() { check(amb_tuning_clear()); });
     m.def("tile_shapes", [](int precision) {
         int n = 0;
         check(amb_tile_count(precision, &n));

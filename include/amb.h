@@ -134,8 +134,21 @@ int amb_set_tile(amb_handle* h, int index);
 
 /* Autotuner for one field: times `reps` steps of every compiled tile shape on a
  * copy of the current state (field and step counter untouched), selects the
- * fastest for this handle and returns its index in `chosen` (nullable). */
+ * fastest for this handle, records it in the tuning table and returns its index in
+ * `chosen` (nullable).  reps < 0: take the tuning table's entry for this precision,
+ * grid and noise mode when there is one (no timing), else time |reps| steps. */
 int amb_autotune(amb_handle* h, int reps, int* chosen);
+
+/* Tuning table (process-wide, no reference counterpart): the autotuners' choices per
+ * (precision, ny, nx, noisy).  amb_create and amb_batch_create start from its entries
+ * (tile option 0), before the occupancy rule.  Entries are plain data, so a caller can
+ * save them and load them into a later process (activemodelbplus.tuning does, as JSON).
+ * amb_tuning_get gives tile = -1 when there is no entry. */
+int amb_tuning_set(int precision, int64_t ny, int64_t nx, int noisy, int tile);
+int amb_tuning_get(int precision, int64_t ny, int64_t nx, int noisy, int* tile);
+int amb_tuning_count(int64_t* count);
+int amb_tuning_entry(int64_t index, int* precision, int64_t* ny, int64_t* nx, int* noisy, int* tile);
+int amb_tuning_clear(void);
 
 /* Thread-local message of the last error on this thread ("" if none). */
 const char* amb_last_error(void);
@@ -202,9 +215,10 @@ int amb_batch_get_tiles(amb_batch* b, int* tile_per_realisation);
 int amb_batch_set_resident(amb_batch* b, int enable);
 int amb_batch_resident_count(amb_batch* b, int64_t* count);
 
-/* Autotuner: for every distinct grid size, times `reps` steps of every compiled
- * tile shape and keeps the fastest (state and step counter are restored).
- * `chosen` (nullable) receives the tile index per realisation. */
+/* Autotuner: for every distinct grid size and noise mode, times `reps` steps of every
+ * compiled tile shape and keeps the fastest (state and step counter are restored),
+ * recording it in the tuning table; reps < 0 takes the table's entries where they
+ * exist.  `chosen` (nullable) receives the tile index per realisation. */
 int amb_batch_autotune(amb_batch* b, int reps, int* chosen);
 
 /* ---- structure factor ------------------------------------------------------ */
