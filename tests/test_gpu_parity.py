@@ -266,12 +266,14 @@ def test_default_tile_per_noise_mode(amb):
     preference: at 4096^2 FP32 the noisy kernel takes 32x48 tiles of 512 threads, the
     noiseless kernel (its 17-tap windows need more registers) 32x24 tiles of 256 threads; a
     1024^2 field one full round of 64x32 tiles either way; fields too small for a round the
-    shape nearest two tiles per CU (32x16 at 512^2, 16x8 at 256^2, in both precisions; at
-    256^2 FP64 the step takes 5.62 us on 16x8 tiles, 5.76 us on 16x16 and 5.93 us on 8x8)."""
+    shape that puts nearest 8 waves on each CU, the earlier in the order on a tie (FP32 at
+    512^2 32x16 tiles of 256 threads, FP64 at 512^2 32x32 tiles of 512 threads, 16x8 of 128
+    threads at 256^2 and 128^2): the fastest shape measured there, or within 2% of it."""
     for precision, side, temperature, want in [(32, 4096, 0.05, (32, 48, 512)), (32, 4096, 0.0, (32, 24, 256)),
                                                (32, 1024, 0.1, (64, 32, 512)), (32, 1024, 0.0, (64, 32, 512)),
                                                (32, 512, 0.05, (32, 16, 256)), (32, 256, 0.0, (16, 8, 128)),
-                                               (64, 256, 0.0, (16, 8, 128))]:
+                                               (64, 256, 0.0, (16, 8, 128)), (64, 512, 0.05, (32, 32, 512)),
+                                               (32, 128, 0.0, (16, 8, 128)), (64, 128, 0.05, (16, 8, 128))]:
         sim = make(amb, np.zeros((side, side)), (5e-3, 1, 1), (-0.25, 0, 0.25, 1, 0.5, -1, temperature),
                    precision=precision)
         shape = tuple(amb.tile_shapes(precision)[sim.options["tile"] - 1])
