@@ -36,7 +36,7 @@ struct BatchDesc {
 // NOISE is kNoiseNone or kNoisePhilox: members with and without noise run in separate
 // launches, so each kernel holds only its own stage-B windows in registers.
 template <typename T, int TW, int TH, int NT, int NOISE>
-__global__ __launch_bounds__(NT, (min_waves_per_eu<T, NT>())) void batch_step_kernel(const BatchDesc<T>* __restrict__ desc,
+__global__ __launch_bounds__(NT, (min_waves_per_eu<T, TW, TH, NT>())) void batch_step_kernel(const BatchDesc<T>* __restrict__ desc,
                                                               const int2* __restrict__ tiles, const T* in, T* out,
                                                               const long long* step_base, int step_off, double* part)
 {

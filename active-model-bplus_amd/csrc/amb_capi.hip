@@ -123,13 +123,16 @@ int validate_params(int64_t ny, int64_t nx, const amb_stencil* st, const amb_mod
 int grid_tiles(int64_t n, int t) { return (int)((n + t - 1) / t); }
 
 // Default tile: in order of preference (AMB_F32_NOISY_ORDER, AMB_F32_NOISELESS_ORDER or
-// AMB_F64_ORDER), the first shape whose tiles fill the device's resident workgroups
-// either for at least 8 rounds or in whole-or-mostly-full rounds (a last round under half
-// full leaves most CUs idle for a round).  Fields too small for one round of any shape take
-// the shape that puts nearest 8 waves on each CU (in ratio; the earlier in the order on a
-// tie): fewer waves leave each CU a single workgroup's latency to hide, while many small
-// tiles pay their halo several times over.  Measured at 128^2, 256^2 and 512^2 in both
-// precisions, with and without noise, this picks the fastest shape or one within 2% of it.
+// AMB_F64_ORDER), the first shape of 256 or more threads whose tiles fill the device's
+// resident workgroups either for at least 8 rounds or in whole-or-mostly-full rounds (a
+// last round under half full leaves most CUs idle for a round).  Fields too small for one
+// round of such a shape take the shape that puts nearest 8 waves on each CU (in ratio; the
+// earlier in the order on a tie): fewer waves leave each CU a single workgroup's latency to
+// hide, while many small tiles pay their halo several times over.  The tiles of 64 and 128
+// threads only serve that way: their boxes hold 4.5-10.6x their area of phi (32x32 tiles
+// 2.4x), so where they fill a round (FP64 16x8 at 512^2) a half round of 32x32 tiles is
+// still faster (9.0 against 10.2 us).  Measured from 128^2 to 4096^2 in both precisions, with and without noise, this
+// picks the fastest shape or one within 2% of it.
 int default_tile(int precision, int64_t ny, int64_t nx, int noise)
 {
     int device = 0, cus = 256;
@@ -158,7 +161,7 @@ int default_tile(int precision, int64_t ny, int64_t nx, int noise)
         const int64_t tiles = (int64_t)grid_tiles(nx, t.tw) * grid_tiles(ny, t.th);
         const int64_t slots = (int64_t)cus * tile_occupancy(precision, i, noise);
         const int64_t full = tiles / slots, rest = tiles % slots;
-        if (full >= 8 || (full >= 1 && (rest == 0 || 2 * rest >= slots))) return i;
+        if (t.nt >= 256 && (full >= 8 || (full >= 1 && (rest == 0 || 2 * rest >= slots)))) return i;
         const double waves = (double)tiles * (t.nt / 64);
         const double gap = std::fabs(std::log(waves / (8.0 * cus)));
         if (k == 0 || gap < best_gap - 1e-9) {

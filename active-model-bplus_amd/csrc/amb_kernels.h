@@ -50,17 +50,17 @@ struct TileShape {
 #define AMB_F32_TILES(X) X(0, 32, 48, 512) X(1, 32, 24, 256) X(2, 32, 32, 256) X(3, 64, 32, 512) X(4, 64, 16, 256) X(5, 32, 16, 256) X(6, 16, 16, 128) X(7, 16, 8, 128) X(8, 8, 8, 64)
 #define AMB_F64_TILES(X) X(0, 32, 32, 256) X(1, 64, 16, 256) X(2, 32, 16, 256) X(3, 32, 32, 512) X(4, 16, 16, 128) X(5, 16, 8, 128) X(6, 8, 8, 64)
 // FP32 orders of preference, measured on MI355X.  Noisy: 32x48 is fastest at 4096^2
-// (113.8 us per step, 116-139 us for the others) and 64x32 at 1024^2, where a single round
-// of 64x32 workgroups pays least halo (11.0 us, 32x32 11.6 us).  Noiseless: the kernel holds
-// its 17-tap windows in registers (92 VGPRs instead of 73), so 256-thread workgroups pack
-// better; 32x24 is fastest at 4096^2 (98.0 us, 32x48 110.4 us) and 64x32 at 1024^2 (9.3 us).
-// 16x16 serves the smallest fields (256^2: 5.3 us noiseless, 32x16 5.6 us).
+// (112.0 us per step, 117-139 us for the others) and 64x32 at 1024^2, where a single round
+// of 64x32 workgroups pays least halo (11.0 us, 32x32 11.4 us).  Noiseless: 32x48 is
+// fastest at 4096^2 too (91.0 us; 32x24 96.5 us, 32x16 98.0 us) and 64x32 at 1024^2
+// (9.2 us, 32x32 9.5 us).  16x8 serves the smallest fields (256^2: 5.2 us noiseless,
+// 16x16 5.3 us, 32x16 5.7 us).
 #define AMB_F32_NOISY_ORDER 0, 3, 2, 1, 4, 5, 6, 7, 8
-#define AMB_F32_NOISELESS_ORDER 1, 3, 2, 5, 0, 4, 6, 7, 8
+#define AMB_F32_NOISELESS_ORDER 0, 3, 1, 5, 2, 4, 6, 7, 8
 // FP64, noisy and noiseless alike: 32x32 tiles of 512 threads (held to 128 VGPRs, so two
-// workgroups, 16 waves, share a CU) are fastest at 4096^2 (294.5 / 182.8 us per step against
-// 324.1 / 208.7 us for 256 threads); 16x8 serves the smallest fields (256^2: 5.64 us, 16x16
-// 5.79 us, 8x8 5.92 us).
+// workgroups, 16 waves, share a CU) are fastest at 4096^2 (291.0 / 178.6 us per step against
+// 321.0 / 213.2 us for 256 threads); 16x8 serves the smallest fields (256^2: 5.56 us, 16x16
+// 5.77 us, 8x8 5.88 us).
 #define AMB_F64_ORDER 3, 0, 2, 1, 4, 5, 6
 
 // Tile shapes compiled per precision (index = autotuner choice).

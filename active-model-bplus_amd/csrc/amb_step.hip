@@ -27,7 +27,7 @@ namespace amb {
 // ---- single-field kernel ----------------------------------------------------------
 
 template <typename T, int TW, int TH, int NT, int NOISE>
-__global__ __launch_bounds__(NT, (min_waves_per_eu<T, NT>())) void step_kernel(const StepArgs<T> p)
+__global__ __launch_bounds__(NT, (min_waves_per_eu<T, TW, TH, NT>())) void step_kernel(const StepArgs<T> p)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     // Tile index; with xcd_chunk the round-robin XCD dispatch gets contiguous tiles.

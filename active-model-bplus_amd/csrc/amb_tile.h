@@ -11,14 +11,6 @@ namespace amb {
 
 constexpr int kThreads = 256;
 
-// Waves per SIMD the register allocation must leave room for: FP64 workgroups of 512
-// threads are held to 128 VGPRs (4 waves per SIMD), so two of them (16 waves) share a CU
-// as their LDS allows; every other kernel is left to the compiler (1 = no constraint).
-template <typename T, int NT>
-constexpr int min_waves_per_eu()
-{
-    return (sizeof(T) == 8 && NT >= 512) ? 4 : 1;
-}
 
 __device__ __forceinline__ int wrap_index(int g, int n)
 {
@@ -33,7 +25,7 @@ struct Lds {
     static constexpr int PW = TW + 18, PH = TH + 18;  // phi: rows [-9,TH+9), cols [-9,TW+9)
     static constexpr int MW = TW + 16, MH = TH + 16;  // mu_p: rows [-8,TH+8), cols [-8,TW+8)
     static constexpr int AW = TW + 4, AH = TH + 4;    // mu_a: [-2, +2)
-    static constexpr int QW = TW + 2, QH = TH + 2;    // circulating current: [-1, +1)
+    static constexpr int QW = TW + 4, QH = TH + 4;    // circulating current: the box [-2, +2)
     static constexpr int XW = TW + 8, XH = TH;        // xi_x: rows [0,TH), cols [-4,TW+4)
     static constexpr int YW = TW, YH = TH + 8;        // xi_y: rows [-4,TH+4), cols [0,TW)
     static constexpr int n_phi = PW * PH;
@@ -53,6 +45,18 @@ struct Lds {
     static_assert(TH % 8 == 0 && NT >= 2 * TW + 12 * (TH / 8), "workgroup must cover the four stage-A arms");
     static_assert(TW % 4 == 0, "noise column groups need TW % 4 == 0");
 };
+
+// Waves per SIMD the register allocation must leave room for: as many as the tile's LDS
+// lets share a CU (160 KB per CU, 4 SIMDs), so that registers never limit occupancy below
+// what LDS allows (e.g. 3 workgroups of 512 threads = 6 waves per SIMD for 32x48 FP32 tiles,
+// 2 of 512 = 4 for 32x32 FP64 tiles).
+template <typename T, int TW, int TH, int NT>
+constexpr int min_waves_per_eu()
+{
+    constexpr int wg = (int)((160 * 1024) / Lds<T, TW, TH, NT>::bytes);
+    constexpr int w = wg * (NT / 64) / 4;
+    return w < 1 ? 1 : w > 8 ? 8 : w;
+}
 
 // ---- Box-Muller ---------------------------------------------------------------
 
@@ -99,16 +103,43 @@ __device__ __forceinline__ T laplacian3(const StepCoef<T>& K, T m0, T xs, T ys, 
 }
 
 template <typename T>
-__device__ __forceinline__ T mu_passive(const StepCoef<T>& K, T m0, T lap)
+__device__ __forceinline__ T mu_passive(const StepCoef<T>& K, T kb, T m0, T lap)
 {
-    return fma(-K.kappa, lap, m0 * fma(m0, fma(m0, K.c, K.b), K.a));
+    return fma(-K.kappa, lap, m0 * fma(m0, fma(m0, K.c, kb), K.a));
+}
+
+// A wave-uniform coefficient held in a vector register: an instruction may read only one
+// scalar register on CDNA, so fma(x, c, b) with both c and b scalar costs a move every time
+// it is evaluated; one move per thread up front instead.  (Empty asm: no instruction.)
+template <typename T>
+__device__ __forceinline__ T in_vgpr(T x)
+{
+    asm volatile("" : "=v"(x) : "0"(x));
+    return x;
+}
+
+// An LDS address as a base register of its own.  The arrays of a tile lie tens of KB into
+// the workgroup's LDS; folded into one constant, such offsets overflow the 8-bit dword
+// offsets of ds_read2 and the compiler spends an add per pair of reads on a fresh base.
+// Held opaque in a register, the base serves every read within 1 KB of it by offset alone.
+// (Empty asm on the 32-bit LDS address: no instruction, no memory access.)
+template <typename T>
+using lds_ptr = __attribute__((address_space(3))) T*;
+
+template <typename T>
+__device__ __forceinline__ lds_ptr<T> lds_base(T* p)
+{
+    uint32_t a = (uint32_t)(uintptr_t)(lds_ptr<T>)p;
+    asm volatile("" : "+v"(a));
+    return (lds_ptr<T>)(uintptr_t)a;
 }
 
 // mu_p at rows r0 .. r0+N-1 of tile column c (tile-relative; LDS phi holds rows and
 // columns from -9, mu_p from -8), marched down: each phi row is read once and its
 // horizontal sum serves the three rows that need it.
 template <typename T, int N, int PW, int MW>
-__device__ __forceinline__ void mu_column(const T* s_phi, T* s_mup, const StepCoef<T>& K, const int r0, const int c)
+__device__ __forceinline__ void mu_column(const T* s_phi, T* s_mup, const StepCoef<T>& K, const T kb, const int r0,
+                                          const int c)
 {
     const T* P = s_phi + (r0 + 8) * PW + c + 8;  // phi at (r0 - 1, c - 1)
     T t0 = P[1], th2 = P[2] + P[0];
@@ -118,7 +149,7 @@ __device__ __forceinline__ void mu_column(const T* s_phi, T* s_mup, const StepCo
     for (int k = 0; k < N; ++k) {
         const T* Q = P + (k + 2) * PW;
         const T b0 = Q[1], bh2 = Q[2] + Q[0];
-        M[k * MW] = mu_passive(K, m0, laplacian3(K, m0, mh2, t0 + b0, th2 + bh2));
+        M[k * MW] = mu_passive(K, kb, m0, laplacian3(K, m0, mh2, t0 + b0, th2 + bh2));
         t0 = m0; th2 = mh2;
         m0 = b0; mh2 = bh2;
     }
@@ -228,32 +259,36 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
         constexpr bool ARM_WAVE = NSEG_RES > 0 && COST_RES < COST_ALL;
         constexpr int SEG = ARM_WAVE ? SEG_RES : SEG_ALL;
         const int seg = tid / BW;
+        const T kb = in_vgpr(K.b);
         if (seg * SEG < BH) {
             const int bc = tid - seg * BW;
             const int br0 = seg * SEG;
-            const T* P = s_phi + (br0 + 6) * L::PW + bc + 6;  // phi at (r0 - 1, c - 1)
+            const auto P = lds_base(s_phi + (br0 + 6) * L::PW + bc + 6);  // phi at (r0 - 1, c - 1)
+            const auto Mw = lds_base(s_mup + (br0 + 6) * L::MW + bc + 6);
+            const auto Aw = lds_base(s_mua + br0 * L::AW + bc);
+            const auto Qxw = lds_base(s_qx + br0 * L::QW + bc);
+            const auto Qyw = lds_base(s_qy + br0 * L::QW + bc);
             T tm = P[0], t0 = P[1], tp = P[2];
             T mm = P[L::PW], m0 = P[L::PW + 1], mp = P[L::PW + 2];
             T th1 = tp - tm, th2 = tp + tm;
             T mh1 = mp - mm, mh2 = mp + mm;
-            const bool qcol = (unsigned)(bc - 1) < (unsigned)(TW + 2);
 #pragma unroll
             for (int k = 0; k < SEG; ++k) {
                 const int br = br0 + k;
-                if (br < BH) {
-                    const T* Q = P + (k + 2) * L::PW;
+                if (BH % SEG == 0 || br < BH) {  // every strip is whole when SEG divides BH
+                    const auto Q = P + (k + 2) * L::PW;
                     const T bm = Q[0], b0 = Q[1], bp = Q[2];
                     const T bh1 = bp - bm, bh2 = bp + bm;
                     const T gx = K.gxe * (th1 + bh1) + K.gxc * mh1;
                     const T gy = K.gye * (bh2 - th2) + K.gyc * (b0 - t0);
                     const T lap = laplacian3(K, m0, mh2, t0 + b0, th2 + bh2);
-                    s_mup[(br + 6) * L::MW + bc + 6] = mu_passive(K, m0, lap);
-                    s_mua[br * L::AW + bc] = K.lam * (gy * gy + gx * gx);
-                    if (qcol && (unsigned)(br - 1) < (unsigned)(TH + 2)) {
-                        const T zl = K.zeta * lap;
-                        s_qx[(br - 1) * L::QW + bc - 1] = zl * gx;
-                        s_qy[(br - 1) * L::QW + bc - 1] = zl * gy;
-                    }
+                    Mw[k * L::MW] = mu_passive(K, kb, m0, lap);
+                    Aw[k * L::AW] = K.lam * (gy * gy + gx * gx);
+                    // the circulating current on the whole box (stage B reads its [-1, T+1)
+                    // core; storing the rim too spares a condition per row)
+                    const T zl = K.zeta * lap;
+                    Qxw[k * L::QW] = zl * gx;
+                    Qyw[k * L::QW] = zl * gy;
                     tm = mm; t0 = m0; tp = mp; th1 = mh1; th2 = mh2;
                     mm = bm; m0 = b0; mp = bp; mh1 = bh1; mh2 = bh2;
                 }
@@ -264,19 +299,19 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
                 const int lane = tid - (NT - 64);
 #pragma unroll
                 for (int a = lane; a < NARM_TB; a += 64)
-                    mu_column<T, 6, L::PW, L::MW>(s_phi, s_mup, K, a < TW ? -8 : TH + 2, a % TW);
+                    mu_column<T, 6, L::PW, L::MW>(s_phi, s_mup, K, kb, a < TW ? -8 : TH + 2, a % TW);
 #pragma unroll
                 for (int a = lane; a < NARM_LR; a += 64) {
                     const int q = a % 12;  // 0..5 left arm, 6..11 right arm
-                    mu_column<T, 8, L::PW, L::MW>(s_phi, s_mup, K, (a / 12) * 8, q < 6 ? q - 8 : TW + 2 + (q - 6));
+                    mu_column<T, 8, L::PW, L::MW>(s_phi, s_mup, K, kb, (a / 12) * 8, q < 6 ? q - 8 : TW + 2 + (q - 6));
                 }
             }
         } else if (tid < NARM_TB) {
-            mu_column<T, 6, L::PW, L::MW>(s_phi, s_mup, K, tid < TW ? -8 : TH + 2, tid % TW);
+            mu_column<T, 6, L::PW, L::MW>(s_phi, s_mup, K, kb, tid < TW ? -8 : TH + 2, tid % TW);
         } else if (tid < NARM_TB + NARM_LR) {
             const int t = tid - NARM_TB;
             const int q = t % 12;
-            mu_column<T, 8, L::PW, L::MW>(s_phi, s_mup, K, (t / 12) * 8, q < 6 ? q - 8 : TW + 2 + (q - 6));
+            mu_column<T, 8, L::PW, L::MW>(s_phi, s_mup, K, kb, (t / 12) * 8, q < 6 ? q - 8 : TW + 2 + (q - 6));
         }
     }
     __syncthreads();
@@ -446,33 +481,40 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
     T mcol[kAlwaysNoisy ? 1 : R + 16];  // mu_p at rows r0-8 .. r0+R+7 of this column
     if constexpr (!kAlwaysNoisy) {
         if (!noisy) {
+            const auto Mc = lds_base(s_mup + r0 * L::MW + c + 8);
 #pragma unroll
-            for (int k = 0; k < R + 16; ++k) mcol[k] = s_mup[(r0 + k) * L::MW + c + 8];
+            for (int k = 0; k < R + 16; ++k) mcol[k] = Mc[k * L::MW];
         }
     }
     T d2x[R + 4], sx[R + 4];  // mu_a rows r0-2 .. r0+R+1
+    const auto Ab = lds_base(s_mua + r0 * L::AW + c + 2);
 #pragma unroll
     for (int k = 0; k < R + 4; ++k) {
-        const T* A = s_mua + (r0 + k) * L::AW + c + 2;
+        const auto A = Ab + k * L::AW;
         const T o2 = A[-2] + A[2];
         d2x[k] = o2 - A[0] - A[0];                                            // (1,0,-2,0,1) along x
         sx[k] = T(kG3G3S[0]) * A[0] + T(kG3G3S[1]) * (A[-1] + A[1]) + T(kG3G3S[2]) * o2;  // (.01,.06,.11,.06,.01)
     }
     T hx[R + 2], sy[R + 2];  // circulating current rows r0-1 .. r0+R
+    const auto Qxb = lds_base(s_qx + (r0 + 1) * L::QW + c + 2);
+    const auto Qyb = lds_base(s_qy + (r0 + 1) * L::QW + c + 2);
 #pragma unroll
     for (int k = 0; k < R + 2; ++k) {
-        const T* Qx = s_qx + (r0 + k) * L::QW + c + 1;
-        const T* Qy = s_qy + (r0 + k) * L::QW + c + 1;
+        const auto Qx = Qxb + k * L::QW;
+        const auto Qy = Qyb + k * L::QW;
         hx[k] = Qx[1] - Qx[-1];
         sy[k] = K.gyc * Qy[0] + K.gye * (Qy[-1] + Qy[1]);
     }
     T ycol[kNeverNoisy ? 1 : R + 8];  // J_y at rows r0-4 .. r0+R+3 of this column
     if constexpr (!kNeverNoisy) {
         if (noisy) {
+            const auto Yc = lds_base(s_jy + r0 * L::YW + c);
 #pragma unroll
-            for (int k = 0; k < R + 8; ++k) ycol[k] = s_jy[(r0 + k) * L::YW + c];
+            for (int k = 0; k < R + 8; ++k) ycol[k] = Yc[k * L::YW];
         }
     }
+    const auto Xb = lds_base(s_jx + r0 * L::XW + c + 4);
+    const auto Mb = lds_base(s_mup + (r0 + 8) * L::MW + c + 8);
     T vout[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
@@ -481,7 +523,7 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
         if constexpr (!kNeverNoisy) {
             if (noisy) {
                 // passive and noise: (1/dx) D8_x J_x + (1/dy) D8_y J_y
-                const T* X = s_jx + r * L::XW + c + 4;
+                const auto X = Xb + k * L::XW;
                 acc = K.dxw[1] * (X[1] - X[-1]);
                 acc = fma(K.dyw[1], ycol[k + 5] - ycol[k + 3], acc);
 #pragma unroll
@@ -494,7 +536,7 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
         if constexpr (!kAlwaysNoisy) {
             if (!noisy) {
                 // passive: -(1/dx^2) D8oD8_x mu_p - (1/dy^2) D8oD8_y mu_p
-                const T* m = s_mup + (r + 8) * L::MW + c + 8;
+                const auto m = Mb + k * L::MW;
                 acc = K.p0 * mcol[k + 8];
 #pragma unroll
                 for (int j = 1; j <= 8; ++j) {

@@ -263,16 +263,17 @@ def test_tile_shapes_bit_identical(amb, temperature):
 
 def test_default_tile_per_noise_mode(amb):
     """Without autotuning the tile comes from the occupancy rule over each kernel's order of
-    preference: at 4096^2 FP32 the noisy kernel takes 32x48 tiles of 512 threads, the
-    noiseless kernel (its 17-tap windows need more registers) 32x24 tiles of 256 threads; a
-    1024^2 field one full round of 64x32 tiles either way; fields too small for a round the
-    shape that puts nearest 8 waves on each CU, the earlier in the order on a tie (FP32 at
-    512^2 32x16 tiles of 256 threads, FP64 at 512^2 32x32 tiles of 512 threads, 16x8 of 128
-    threads at 256^2 and 128^2): the fastest shape measured there, or within 2% of it."""
-    for precision, side, temperature, want in [(32, 4096, 0.05, (32, 48, 512)), (32, 4096, 0.0, (32, 24, 256)),
+    preference: at 4096^2 FP32 both kernels take 32x48 tiles of 512 threads; a 1024^2 field
+    one full round of 64x32 tiles either way, FP64 at 1024^2 two rounds of 32x32 tiles of
+    512 threads; fields too small for a round the shape that puts nearest 8 waves on each
+    CU, the earlier in the order on a tie (FP32 at 512^2 32x16 tiles of 256 threads, FP64
+    at 512^2 32x32 tiles of 512 threads, 16x8 of 128 threads at 256^2 and 128^2): the
+    fastest shape measured there, or within 2% of it."""
+    for precision, side, temperature, want in [(32, 4096, 0.05, (32, 48, 512)), (32, 4096, 0.0, (32, 48, 512)),
                                                (32, 1024, 0.1, (64, 32, 512)), (32, 1024, 0.0, (64, 32, 512)),
                                                (32, 512, 0.05, (32, 16, 256)), (32, 256, 0.0, (16, 8, 128)),
                                                (64, 256, 0.0, (16, 8, 128)), (64, 512, 0.05, (32, 32, 512)),
+                                               (64, 512, 0.0, (32, 32, 512)), (64, 1024, 0.05, (32, 32, 512)),
                                                (32, 128, 0.0, (16, 8, 128)), (64, 128, 0.05, (16, 8, 128))]:
         sim = make(amb, np.zeros((side, side)), (5e-3, 1, 1), (-0.25, 0, 0.25, 1, 0.5, -1, temperature),
                    precision=precision)
