@@ -47,16 +47,18 @@ struct TileShape {
 
 // Compiled tile configurations (index = autotuner choice): X(index, TW, TH, NT).  The
 // default tile follows an order of preference per kernel (default_tile in amb_capi.hip).
-#define AMB_F32_TILES(X) X(0, 32, 48, 512) X(1, 32, 24, 256) X(2, 32, 32, 256) X(3, 64, 32, 512) X(4, 64, 16, 256) X(5, 32, 16, 256) X(6, 16, 16, 128) X(7, 16, 8, 128) X(8, 8, 8, 64)
+#define AMB_F32_TILES(X) X(0, 32, 48, 512) X(1, 32, 24, 256) X(2, 32, 32, 256) X(3, 64, 32, 512) X(4, 64, 16, 256) X(5, 32, 16, 256) X(6, 16, 16, 128) X(7, 16, 8, 128) X(8, 8, 8, 64) X(9, 32, 32, 512)
 #define AMB_F64_TILES(X) X(0, 32, 32, 256) X(1, 64, 16, 256) X(2, 32, 16, 256) X(3, 32, 32, 512) X(4, 16, 16, 128) X(5, 16, 8, 128) X(6, 8, 8, 64)
-// FP32 orders of preference, measured on MI355X.  Noisy: 32x48 is fastest at 4096^2
-// (112.0 us per step, 117-139 us for the others) and 64x32 at 1024^2, where a single round
-// of 64x32 workgroups pays least halo (11.0 us, 32x32 11.4 us).  Noiseless: 32x48 is
-// fastest at 4096^2 too (91.0 us; 32x24 96.5 us, 32x16 98.0 us) and 64x32 at 1024^2
-// (9.2 us, 32x32 9.5 us).  16x8 serves the smallest fields (256^2: 5.2 us noiseless,
-// 16x16 5.3 us, 32x16 5.7 us).
-#define AMB_F32_NOISY_ORDER 0, 3, 2, 1, 4, 5, 6, 7, 8
-#define AMB_F32_NOISELESS_ORDER 0, 3, 1, 5, 2, 4, 6, 7, 8
+// FP32 orders of preference, measured on MI355X.  Noisy: 32x32 tiles of 512 threads are
+// fastest at 4096^2 (108.1 us per step; 32x48 111.5 us, 32x16 116.9 us, the others 120-139
+// us): their 35 KB of LDS and 61 VGPRs let four workgroups, 8 waves per SIMD, share a CU,
+// against three for 32x48, and the extra halo work costs less than the extra waves hide;
+// at 1024^2 they tie with 64x32 (11.02 / 11.04 us).  Noiseless: 32x48 is fastest at 4096^2
+// (90.6 us; 32x32x512 93.1 us, whose 17-tap windows spill at 64 VGPRs; 32x24 96.0 us) and
+// 64x32 at 1024^2 (9.2 us, 32x32 9.5 us).  16x8 serves the smallest fields (256^2: 5.1 us
+// noiseless, 6.2 us noisy, 8x8 6.0 us).
+#define AMB_F32_NOISY_ORDER 9, 0, 3, 2, 1, 4, 5, 6, 7, 8
+#define AMB_F32_NOISELESS_ORDER 0, 3, 9, 1, 5, 2, 4, 6, 7, 8
 // FP64, noisy and noiseless alike: 32x32 tiles of 512 threads (held to 128 VGPRs, so two
 // workgroups, 16 waves, share a CU) are fastest at 4096^2 (291.0 / 178.6 us per step against
 // 321.0 / 213.2 us for 256 threads); 16x8 serves the smallest fields (256^2: 5.56 us, 16x16

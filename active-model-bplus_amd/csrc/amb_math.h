@@ -88,56 +88,106 @@ __host__ __device__ inline U4 noise_counter(uint32_t group, uint32_t row, uint64
     return c;
 }
 
-// Mantissa in [1/2, 1) and exponent of a finite non-zero double (two instructions on gfx950).
-__host__ __device__ inline double frexp_mant(double u, int* e)
+// FP64 Box-Muller on the two uniforms a Philox word pair gives.  The generic math library
+// handles every input (zeros, infinities, huge arguments, denormals) with special-case code
+// that dominated the FP64 noisy step; here the inputs are known, so short kernels of our own
+// suffice, all accurate to a few ulp:
+//   -2 ln u for u = (w + 1/2) 2^-32 in [2^-33, 1): u = m 2^e with m in [sqrt(1/2), sqrt(2))
+//     (e from the exponent of u sqrt(2)), f = m - 1 (exact), s = f / (m + 1), |s| <= 3 - 2
+//     sqrt(2) = 0.1716, ln m = 2 atanh(s) = 2 s + 2 s^3 g(s^2) with g a 7-term polynomial;
+//     e ln 2 with ln 2 split in a high part whose products with |e| <= 34 are exact and a
+//     low part;
+//   sin and cos of 2 pi u for u = w 2^-32 in [0, 1): quadrant j = rint(4u), reduced
+//     argument x = (4u - j) pi/2 in [-pi/4, pi/4] (4u - j is exact), sin x = x + x^3 S(x^2),
+//     cos x = 1 - x^2/2 + x^4 C(x^2) with 6-term polynomials S and C, then the quadrant's
+//     swap and signs;
+//   the quotient s and the square root of -2 ln u (a normal number in [2.3e-10, 46.1]) by
+//     Newton steps from the gfx950 reciprocal and reciprocal-square-root instructions.
+// g, S and C are Chebyshev fits at 60 digits (noise_poly_fit.py prints them and their
+// worst relative errors over their intervals: 4.7e-16, 2.0e-16, 5.7e-17).
+constexpr double kLnG[7] = {0x1.2b584aae78a57p-4, 0x1.39fe606542ddep-4, 0x1.7462b4ab2ef6bp-4, 0x1.c71c62e5800a1p-4,
+                            0x1.2492492df148dp-3, 0x1.99999999952e2p-3, 0x1.5555555555558p-2};
+constexpr double kSinS[6] = {0x1.5e0b19f8b1451p-33, -0x1.ae600b02b6262p-26, 0x1.71de37968a100p-19,
+                             -0x1.a01a019e83aaep-13, 0x1.1111111110bb2p-7,  -0x1.5555555555555p-3};
+constexpr double kCosC[6] = {-0x1.907da367a37cbp-37, 0x1.1eeb68e93b64cp-29, -0x1.27e4fa17da09ep-22,
+                             0x1.a01a019f4eb01p-16,  -0x1.6c16c16c16967p-10, 0x1.5555555555555p-5};
+
+// Exponent e of a finite non-zero double, x = m 2^e with m in [1/2, 1); x 2^k.
+__host__ __device__ inline int exponent_of(double x)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-    *e = __builtin_amdgcn_frexp_exp(u);
-    return __builtin_amdgcn_frexp_mant(u);
+    return __builtin_amdgcn_frexp_exp(x);
 #else
-    return std::frexp(u, e);
+    int e;
+    (void)std::frexp(x, &e);
+    return e;
 #endif
 }
 
-// FP64 Box-Muller on the two uniforms a Philox word pair gives.  The generic math library
-// handles every input (zeros, infinities, huge arguments, denormals) with special-case code
-// that dominated the FP64 noisy step; here the inputs are known, so two short kernels of
-// our own suffice, both accurate to a few ulp:
-//   ln u for u = (w + 1/2) 2^-32 in [2^-33, 1): u = m 2^e with m in [sqrt(1/2), sqrt(2)),
-//     ln m = 2 atanh(s), s = f / (2 + f), f = m - 1 (exact), |s| <= 0.1716, Taylor series
-//     of atanh to s^21 (truncation < 1e-18 relative); e ln 2 with ln 2 split in a high
-//     part whose products with |e| <= 34 are exact and a low part;
-//   sin and cos of 2 pi u for u = w 2^-32 in [0, 1): quadrant j = rint(4u), reduced
-//     argument x = (4u - j) pi/2 in [-pi/4, pi/4] (4u - j is exact), Taylor series of sin
-//     to x^17 and of cos to x^16 (truncation < 1e-16 relative), then the quadrant's
-//     swap and signs.
-__host__ __device__ inline double ln_unit(double u)
+__host__ __device__ inline double scale2(double x, int k)
 {
-    int e;
-    double m = frexp_mant(u, &e);  // u = m 2^e, m in [1/2, 1)
-    if (m < 0.70710678118654752440) {
-        m *= 2.0;
-        e -= 1;
-    }
-    const double f = m - 1.0;
-    const double s = f / (2.0 + f);
-    const double s2 = s * s;
-    double p = 1.0 / 21.0;
-    p = fma(p, s2, 1.0 / 19.0);
-    p = fma(p, s2, 1.0 / 17.0);
-    p = fma(p, s2, 1.0 / 15.0);
-    p = fma(p, s2, 1.0 / 13.0);
-    p = fma(p, s2, 1.0 / 11.0);
-    p = fma(p, s2, 1.0 / 9.0);
-    p = fma(p, s2, 1.0 / 7.0);
-    p = fma(p, s2, 1.0 / 5.0);
-    p = fma(p, s2, 1.0 / 3.0);
-    const double ts = 2.0 * s;
-    const double lnm = fma(ts * s2, p, ts);  // 2 s (1 + s^2 p)
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_ldexp(x, k);
+#else
+    return std::ldexp(x, k);
+#endif
+}
+
+// Seeds for the Newton steps: v_rcp_f64 and v_rsq_f64 on the device, the correctly
+// rounded values on the host (the steps below bring either to the same few-ulp result).
+__host__ __device__ inline double rcp_seed(double d)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcp(d);
+#else
+    return 1.0 / d;
+#endif
+}
+
+__host__ __device__ inline double rsq_seed(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rsq(x);
+#else
+    return 1.0 / std::sqrt(x);
+#endif
+}
+
+// -2 ln u for u in [2^-33, 1).
+__host__ __device__ inline double neg2_ln_unit(double u)
+{
+    const int e = exponent_of(u * 1.41421356237309504880) - 1;
+    const double m = scale2(u, -e);  // exact, in [sqrt(1/2), sqrt(2)) up to the rounding of u sqrt(2)
+    const double f = m - 1.0;        // exact
+    const double d = 2.0 + f;
+    double r = rcp_seed(d);
+    r = fma(r, fma(-d, r, 1.0), r);  // one Newton step on 1/d
+    double s = f * r;
+    s = fma(fma(-d, s, f), r, s);    // one residual correction of f/d
+    const double t = s * s;
+    double p = kLnG[0];
+#pragma unroll
+    for (int k = 1; k < 7; ++k) p = fma(p, t, kLnG[k]);
+    const double q = -4.0 * s;
+    const double lnm2 = fma(q * t, p, q);  // -2 ln m = -4 s (1 + s^2 g(s^2))
     const double de = (double)e;
     constexpr double kLn2Hi = 6.93147180369123816490e-01;  // 32 significant bits: de * kLn2Hi is exact
     constexpr double kLn2Lo = 1.90821492927058770002e-10;
-    return fma(de, kLn2Hi, fma(de, kLn2Lo, lnm));
+    return fma(de, -2.0 * kLn2Hi, fma(de, -2.0 * kLn2Lo, lnm2));
+}
+
+// ln u for u in [2^-33, 1): the value the kernels use, halved back (exact).
+__host__ __device__ inline double ln_unit(double u) { return -0.5 * neg2_ln_unit(u); }
+
+// sqrt x for a normal x well inside the double range.
+__host__ __device__ inline double sqrt_normal(double x)
+{
+    const double y = rsq_seed(x);
+    double g = x * y, h = 0.5 * y;
+    const double e = fma(-g, h, 0.5);
+    g = fma(g, e, g);  // sqrt x
+    h = fma(h, e, h);  // 1 / (2 sqrt x)
+    return fma(fma(-g, g, x), h, g);
 }
 
 __host__ __device__ inline void sincos_turn(double u, double& s, double& c)
@@ -146,22 +196,12 @@ __host__ __device__ inline void sincos_turn(double u, double& s, double& c)
     const double j = rint(q);
     const double x = (q - j) * 1.57079632679489661923;  // (4u - j) pi/2
     const double x2 = x * x;
-    // sin x = x + x^3 S(x^2), cos x = 1 - x^2/2 + x^4 C(x^2)
-    double sp = 1.0 / 355687428096000.0;                // 1/17!
-    sp = fma(sp, x2, -1.0 / 1307674368000.0);           // -1/15!
-    sp = fma(sp, x2, 1.0 / 6227020800.0);               // 1/13!
-    sp = fma(sp, x2, -1.0 / 39916800.0);                // -1/11!
-    sp = fma(sp, x2, 1.0 / 362880.0);                   // 1/9!
-    sp = fma(sp, x2, -1.0 / 5040.0);                    // -1/7!
-    sp = fma(sp, x2, 1.0 / 120.0);                      // 1/5!
-    sp = fma(sp, x2, -1.0 / 6.0);                       // -1/3!
-    double cp = 1.0 / 20922789888000.0;                 // 1/16!
-    cp = fma(cp, x2, -1.0 / 87178291200.0);             // -1/14!
-    cp = fma(cp, x2, 1.0 / 479001600.0);                // 1/12!
-    cp = fma(cp, x2, -1.0 / 3628800.0);                 // -1/10!
-    cp = fma(cp, x2, 1.0 / 40320.0);                    // 1/8!
-    cp = fma(cp, x2, -1.0 / 720.0);                     // -1/6!
-    cp = fma(cp, x2, 1.0 / 24.0);                       // 1/4!
+    double sp = kSinS[0], cp = kCosC[0];
+#pragma unroll
+    for (int k = 1; k < 6; ++k) {
+        sp = fma(sp, x2, kSinS[k]);
+        cp = fma(cp, x2, kCosC[k]);
+    }
     const double sx = fma(x * x2, sp, x);
     const double cx = fma(x2 * x2, cp, fma(-0.5, x2, 1.0));
     const int k = (int)j & 3;
@@ -175,7 +215,7 @@ __host__ __device__ inline void box_muller(uint32_t wa, uint32_t wb, double& z0,
 {
     const double u1 = ((double)wa + 0.5) * 0x1p-32;
     const double u2 = (double)wb * 0x1p-32;
-    const double r = sqrt(-2.0 * ln_unit(u1));
+    const double r = sqrt_normal(neg2_ln_unit(u1));
     double s, c;
     sincos_turn(u2, s, c);  // sin and cos of 2 pi u2
     z0 = r * c;

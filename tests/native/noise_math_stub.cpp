@@ -1,6 +1,8 @@
 // Host build of the device noise arithmetic (active-model-bplus_amd/csrc/amb_math.h) for the
-// CPU tests: Philox4x32-10 plus the FP64 Box-Muller (own logarithm and sin/cos) exactly as
-// the kernels evaluate them.  Built by tests/test_noise_math.py with g++.
+// CPU tests: Philox4x32-10 plus the FP64 Box-Muller (own logarithm, square root and sin/cos)
+// as the kernels evaluate them, except for the seeds of the Newton steps (the correctly
+// rounded reciprocal and reciprocal square root here, the gfx950 instructions there).
+// Built by tests/test_noise_math.py with g++.
 #define __host__
 #define __device__
 #include <cstdint>
