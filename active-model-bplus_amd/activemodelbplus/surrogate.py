@@ -30,7 +30,8 @@ DELTA_STEPS = 100
 
 # ---- observables --------------------------------------------------------------------
 # The structure factor follows examples/structureFactor.ipynb cell 7 (see trajectory.py).
-from .trajectory import BIN_EDGES, mean_wavenumber, peak_wavenumber, structure_factor  # noqa: E402,F401
+from .trajectory import (BIN_EDGES, mean_wavenumber, peak_wavenumber, smoothed_peak_wavenumber,  # noqa: E402,F401
+                         structure_factor)
 
 
 # ---- data -----------------------------------------------------------------------------
@@ -243,13 +244,16 @@ class HybridIntegrator:
 # ---- evaluation ------------------------------------------------------------------------
 
 def evaluate(sur, n_eval=8, side=256, t_max=1e4, seed=1234, audit_every=10, tol=0.02, checkpoints=(10, 100, 1000, 1e4),
-             log=print, euler_timing=True, max_audit_every=200):
+             log=print, euler_timing=True, max_audit_every=200, fields_out=None):
     """Held-out parameter draws: Euler reference vs hybrid vs surrogate-only vs coarse Euler.
 
-    Returns a dict of metrics: S(k) first moment and peak at each checkpoint time, mass
-    drift, wall times and the speedup of the hybrid over plain Euler to t_max.  A second
-    Euler realisation per member (same parameters, other initial noise) gives the
-    natural scatter of <k> against which the surrogate's error is read."""
+    Returns a dict of metrics: S(k) first moment, argmax bin and smoothed peak
+    (trajectory.smoothed_peak_wavenumber) at each checkpoint time, mass drift, wall times
+    and the speedup of the hybrid over plain Euler to t_max.  A second Euler realisation per
+    member (same parameters, other initial noise) gives the natural scatter of <k> and of
+    the peak against which the surrogate's error is read, and an FP64 Euler run of the same
+    initial field the divergence that a change of arithmetic alone produces by t_max.
+    ``fields_out`` (a path) keeps the fields at t_max in an .npz."""
     rng = np.random.default_rng(seed)
     params = sample_params(rng, n_eval)
     # keep held-out draws inside the spinodal region so that they phase-separate
@@ -273,14 +277,27 @@ def evaluate(sur, n_eval=8, side=256, t_max=1e4, seed=1234, audit_every=10, tol=
         ref_fields[c] = [ens.field(i) for i in range(n_eval)]
         twin_fields[c] = [ens.field(n_eval + i) for i in range(n_eval)]
         log(f"  euler reference at t={c:g}")
+    del ens
+    # the same initial fields in FP64 (the pointwise divergence of two arithmetics of one
+    # deterministic, chaotic coarsening: the floor for any per-member comparison)
+    ens64 = Ensemble(fields, st, models, list(range(n_eval)), precision=64)
+    ens64.run(int(round(cps[-1] / DT)))
+    fp64_fields = [ens64.field(i) for i in range(n_eval)]
+    del ens64
+    res["euler_fp64"] = [dict(k_mean=mean_wavenumber(f), k_peak_smooth=smoothed_peak_wavenumber(f)) for f in fp64_fields]
+    log("  euler FP64 reference done")
     res["euler_twin_k_mean"] = [{str(c): mean_wavenumber(twin_fields[c][i]) for c in cps} for i in range(n_eval)]
+    res["euler_twin_k_peak_smooth"] = [{str(c): smoothed_peak_wavenumber(twin_fields[c][i]) for c in cps}
+                                       for i in range(n_eval)]
     for i in range(n_eval):
         res["euler"].append({str(c): dict(k_mean=mean_wavenumber(ref_fields[c][i]),
                                          k_peak=peak_wavenumber(ref_fields[c][i]),
+                                         k_peak_smooth=smoothed_peak_wavenumber(ref_fields[c][i]),
                                          mass=float(ref_fields[c][i].sum())) for c in cps})
 
     # hybrid and surrogate-only rollouts
     t_hybrid = []
+    finals = {}
     for i in range(n_eval):
         for mode in ("hybrid", "surrogate_only"):
             if mode == "hybrid":
@@ -297,8 +314,10 @@ def evaluate(sur, n_eval=8, side=256, t_max=1e4, seed=1234, audit_every=10, tol=
                 wall += time.perf_counter() - t0  # rollout only, not the analysis below
                 prev = c
                 out[str(c)] = dict(k_mean=mean_wavenumber(f), k_peak=peak_wavenumber(f),
+                                   k_peak_smooth=smoothed_peak_wavenumber(f),
                                    mass_drift=float(f.sum() - m0), l2_vs_euler=float(
                                        np.linalg.norm(f - ref_fields[c][i]) / np.linalg.norm(ref_fields[c][i])))
+            finals.setdefault(mode, []).append(h.field.copy())
             stats = {k: v for k, v in h.stats.items() if k != "audit_errors"}
             stats["median_audit_error"] = float(np.median(h.stats["audit_errors"])) if h.stats["audit_errors"] else None
             res[mode].append(dict(metrics=out, wall_s=wall, stats=stats))
@@ -340,6 +359,28 @@ def evaluate(sur, n_eval=8, side=256, t_max=1e4, seed=1234, audit_every=10, tol=
                                          for i in range(n_eval)]
     res["k_peak_rel_err_hybrid"] = [res["hybrid"][i]["metrics"][last]["k_peak"] / res["euler"][i][last]["k_peak"] - 1
                                     for i in range(n_eval)]
+    # the structure-factor peak (trajectory.smoothed_peak_wavenumber), per member, for the
+    # hybrid, the surrogate alone, the second Euler realisation and the FP64 Euler run of the
+    # same initial field (the floor that different arithmetic alone reaches)
+    ref_pk = [res["euler"][i][last]["k_peak_smooth"] for i in range(n_eval)]
+    for key, vals in (("hybrid", [res["hybrid"][i]["metrics"][last]["k_peak_smooth"] for i in range(n_eval)]),
+                      ("surrogate_only",
+                       [res["surrogate_only"][i]["metrics"][last]["k_peak_smooth"] for i in range(n_eval)]),
+                      ("euler_twin", [res["euler_twin_k_peak_smooth"][i][last] for i in range(n_eval)]),
+                      ("euler_fp64", [res["euler_fp64"][i]["k_peak_smooth"] for i in range(n_eval)])):
+        errs = [v / r - 1 for v, r in zip(vals, ref_pk)]
+        res[f"k_peak_smooth_rel_err_{key}"] = errs
+        res[f"k_peak_smooth_rel_err_{key}_mean"] = float(np.mean(errs))
+        res[f"k_peak_smooth_rel_err_{key}_rms"] = float(np.sqrt(np.mean(np.square(errs))))
+        res[f"k_peak_smooth_rel_err_{key}_max_abs"] = float(np.max(np.abs(errs)))
+    res["k_mean_rel_err_euler_fp64"] = [res["euler_fp64"][i]["k_mean"] / res["euler"][i][last]["k_mean"] - 1
+                                        for i in range(n_eval)]
+    if fields_out:
+        # the fields at t_max, for analysis of other observables
+        np.savez_compressed(fields_out, params=params, euler=np.asarray(ref_fields[cps[-1]]),
+                            euler_twin=np.asarray(twin_fields[cps[-1]]), euler_fp64=np.asarray(fp64_fields),
+                            hybrid=np.asarray(finals["hybrid"]),
+                            surrogate_only=np.asarray(finals["surrogate_only"]))
     return res
 
 
@@ -368,6 +409,7 @@ def _main(argv=None):
     e.add_argument("--max-audit-every", type=int, default=200)
     e.add_argument("--tol", type=float, default=0.02)
     e.add_argument("--out")
+    e.add_argument("--fields", help="write the fields at t_max to this .npz")
     a = p.parse_args(argv)
 
     def log(msg):
@@ -415,7 +457,7 @@ def _main(argv=None):
         sur = Surrogate(256, 256, max_batch=1, seed=0)
         meta = load_weights(sur, a.weights)
         res = evaluate(sur, a.members, 256, a.t_max, audit_every=a.audit_every, tol=a.tol, log=log,
-                       max_audit_every=a.max_audit_every)
+                       max_audit_every=a.max_audit_every, fields_out=a.fields)
         res["weights_meta"] = meta
         text = json.dumps(res)
         if a.out:

@@ -355,6 +355,7 @@ def bench_surrogate(args):
         "fallback_coarse_steps": s["fallback_steps"], "audits": s["audits"],
         "max_audit_error": s["max_audit_error"],
         "k_mean_euler": k_e, "k_mean_rel_err": k_h / k_e - 1,
+        "k_peak_smooth_rel_err": SG.smoothed_peak_wavenumber(f_h) / SG.smoothed_peak_wavenumber(f_e) - 1,
         "l2_rel_err_vs_euler": float(np.linalg.norm(f_h - f_e) / np.linalg.norm(f_e)),
         "mass_drift": float(f_h.sum() - phi.astype(np.float32).sum()),
         "roofline": {"bound": "mfma", "achieved": tflops, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",

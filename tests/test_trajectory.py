@@ -79,3 +79,24 @@ def test_trajectory_file_layout(tmp_path):
     assert set(z) == {"time", "field", "a", "b", "c", "kappa", "lamb", "zeta", "temperature", "dt", "dx", "dy"}
     np.testing.assert_array_equal(z["field"], fields)
     assert float(z["lamb"]) == 0.5 and float(z["dt"]) == 1e-2
+
+
+def test_smoothed_peak_of_a_ring_and_its_invariances():
+    """Power on the modes of one lattice ring |k| = k0 (plus weaker modes at 2 k0): the
+    smoothed maximum lies within a tenth of a mode spacing of k0, also with a kernel twice as
+    wide (the 1/|k| weight removes the pull of the growing mode density).  Amplitude scaling,
+    a constant offset and a lattice translation leave it unchanged."""
+    n = 256
+    y, x = np.mgrid[0:n, 0:n]
+    m = 5  # k0 = 2 pi 5 / 256 = 0.123: the lattice vectors of length 5
+    phi = np.zeros((n, n))
+    for a, b in [(5, 0), (0, 5), (3, 4), (4, 3), (3, -4), (4, -3)]:
+        phi += np.cos(2 * np.pi * (a * x + b * y) / n + 0.3 * a)
+    phi += 0.3 * np.cos(2 * np.pi * 10 * x / n) + 0.3 * np.cos(2 * np.pi * (6 * x + 8 * y) / n)
+    k0 = 2 * np.pi * m / n
+    w = 2 * np.pi / n
+    kp = T.smoothed_peak_wavenumber(phi)
+    assert abs(kp - k0) < 0.1 * w, (kp, k0)
+    assert T.smoothed_peak_wavenumber(3.7 * phi + 0.2) == pytest.approx(kp, rel=1e-12)
+    assert T.smoothed_peak_wavenumber(np.roll(phi, (17, -5), axis=(0, 1))) == pytest.approx(kp, rel=1e-9)
+    assert abs(T.smoothed_peak_wavenumber(phi, width=2 * w) - k0) < 0.1 * w
