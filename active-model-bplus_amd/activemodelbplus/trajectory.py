@@ -112,11 +112,11 @@ def smoothed_peak_wavenumber(field, width=None, dx=1.0, dy=1.0):
     h = width / 50
     keep = K > 0
     idx = np.rint(K[keep] / h).astype(np.int64)
-    n = int(idx.max()) + 1 + 300
+    t = np.arange(-300, 301) * h
+    n = max(int(idx.max()) + 1 + 300, t.size)  # >= the kernel's length, so "same" keeps n bins
     inv = 1.0 / K[keep]
     sums = np.bincount(idx, weights=p[keep] * inv, minlength=n)
     counts = np.bincount(idx, weights=inv, minlength=n)
-    t = np.arange(-300, 301) * h
     g = np.exp(-0.5 * (t / width) ** 2)
     num = np.convolve(sums, g, mode="same")
     den = np.convolve(counts, g, mode="same")

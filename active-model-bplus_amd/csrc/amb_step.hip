@@ -396,6 +396,8 @@ __global__ __launch_bounds__(kTileReduceThreads) void reduce_tiles_kernel(const 
     __shared__ double ssum[kTileReduceThreads];
     __shared__ double sbad[kTileReduceThreads];
     constexpr int kBatch = 16;
+    // entry i as one 16-byte load (every partials array starts 16-byte aligned: hipMalloc,
+    // or 256 bytes into the no-allocation entry point's 256-byte aligned scratch array)
     const double2* p2 = reinterpret_cast<const double2*>(part);
     double s = 0.0, b = 0.0;
     for (int64_t i0 = threadIdx.x; i0 < n; i0 += (int64_t)kBatch * kTileReduceThreads) {

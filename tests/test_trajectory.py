@@ -100,3 +100,23 @@ def test_smoothed_peak_of_a_ring_and_its_invariances():
     assert T.smoothed_peak_wavenumber(3.7 * phi + 0.2) == pytest.approx(kp, rel=1e-12)
     assert T.smoothed_peak_wavenumber(np.roll(phi, (17, -5), axis=(0, 1))) == pytest.approx(kp, rel=1e-9)
     assert abs(T.smoothed_peak_wavenumber(phi, width=2 * w) - k0) < 0.1 * w
+
+
+def test_smoothed_peak_matches_a_direct_evaluation():
+    """The binned convolution equals the smoothed curve written out mode by mode, on grids
+    where the kernel is longer than the spectrum (8x8) and shorter (24x40, anisotropic): the
+    same maximum to the fine grid's resolution."""
+    rng = np.random.default_rng(3)
+    for shape, dx, dy in [((8, 8), 1.0, 1.0), ((24, 40), 0.75, 1.25)]:
+        phi = rng.normal(size=shape)
+        ny, nx = shape
+        p = (np.abs(np.fft.fft2(phi - phi.mean()) * dx * dy) ** 2).ravel()
+        K = np.sqrt((2 * np.pi * np.fft.fftfreq(nx, d=dx))[None, :] ** 2
+                    + (2 * np.pi * np.fft.fftfreq(ny, d=dy))[:, None] ** 2).ravel()
+        keep = K > 0
+        width = max(2 * np.pi / (nx * dx), 2 * np.pi / (ny * dy))
+        grid = np.arange(0, K.max() + 6 * width, width / 50)
+        w = np.exp(-0.5 * ((grid[:, None] - K[keep][None, :]) / width) ** 2) / K[keep][None, :]
+        S = (w @ p[keep]) / w.sum(axis=1)
+        k_direct = grid[np.argmax(S)]
+        assert abs(T.smoothed_peak_wavenumber(phi, dx=dx, dy=dy) - k_direct) < 1.5 * width / 50
