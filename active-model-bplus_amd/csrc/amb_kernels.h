@@ -49,7 +49,8 @@ struct TileShape {
 // default tile follows an order of preference per kernel (default_tile in amb_capi.hip).
 #define AMB_F32_TILES(X) X(0, 32, 48, 512) X(1, 32, 24, 256) X(2, 32, 32, 256) X(3, 64, 32, 512) X(4, 64, 16, 256) X(5, 32, 16, 256) X(6, 16, 16, 128) X(7, 16, 8, 128) X(8, 8, 8, 64) X(9, 32, 32, 512)
 #define AMB_F64_TILES(X) X(0, 32, 32, 256) X(1, 64, 16, 256) X(2, 32, 16, 256) X(3, 32, 32, 512) X(4, 16, 16, 128) X(5, 16, 8, 128) X(6, 8, 8, 64)
-// FP32 orders of preference, measured on MI355X.  Noisy: 32x32 tiles of 512 threads are
+// FP32 orders of preference, measured on MI355X (the times below predate the step's wave
+// priorities, which took 2-4% off every shape).  Noisy: 32x32 tiles of 512 threads are
 // fastest at 4096^2 (108.1 us per step; 32x48 111.5 us, 32x16 116.9 us, the others 120-139
 // us): their 35 KB of LDS and 61 VGPRs let four workgroups, 8 waves per SIMD, share a CU,
 // against three for 32x48, and the extra halo work costs less than the extra waves hide;
@@ -61,7 +62,7 @@ struct TileShape {
 #define AMB_F32_NOISELESS_ORDER 0, 3, 9, 1, 5, 2, 4, 6, 7, 8
 // FP64, noisy and noiseless alike: 32x32 tiles of 512 threads (held to 128 VGPRs, so two
 // workgroups, 16 waves, share a CU) are fastest at 4096^2 (291.0 / 178.6 us per step against
-// 321.0 / 213.2 us for 256 threads); 16x8 serves the smallest fields (256^2: 5.56 us, 16x16
+// 321.0 / 213.2 us for 256 threads, both before the fitted noise polynomials and the priorities); 16x8 serves the smallest fields (256^2: 5.56 us, 16x16
 // 5.77 us, 8x8 5.88 us).
 #define AMB_F64_ORDER 3, 0, 2, 1, 4, 5, 6
 
