@@ -89,8 +89,9 @@ hipError_t launch_observable_f64(const StepArgs<double>& a, int which, double* t
 hipError_t launch_reduce_tiles(const double* part, int64_t ntiles, double* sum, unsigned long long* bad,
                                hipStream_t s);
 
-// FP64 sum and non-finite count of a field: deterministic two-pass reduction.
-constexpr int kReduceBlocks = 512;
+// FP64 sum and non-finite count of a field: deterministic two-pass reduction (at most
+// kReduceBlocks partials; their number depends only on the field's size).
+constexpr int kReduceBlocks = 2048;
 hipError_t launch_reduce_f32(const float* phi, int64_t n, double* part, unsigned long long* bad_part,
                              double* sum, unsigned long long* bad, hipStream_t s);
 hipError_t launch_reduce_f64(const double* phi, int64_t n, double* part, unsigned long long* bad_part,

@@ -337,9 +337,24 @@ __global__ __launch_bounds__(kThreads) void reduce_partial_kernel(const T* phi, 
     __shared__ unsigned long long sbad[kThreads];
     double s = 0.0;
     unsigned long long b = 0;
-    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-        const double v = (double)phi[i];
-        if (isfinite(v)) s += v; else ++b;
+    // Each thread visits i0, i0 + G, i0 + 2G, ... (G = all threads of the grid) and adds in
+    // that order; the values are fetched eight at a time first, so the loads overlap.
+    constexpr int kBatch = 8;
+    const int64_t G = (int64_t)gridDim.x * kThreads;
+    for (int64_t i0 = (int64_t)blockIdx.x * kThreads + threadIdx.x; i0 < n; i0 += kBatch * G) {
+        T e[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int64_t i = i0 + u * G;
+            e[u] = i < n ? phi[i] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            if (i0 + u * G < n) {
+                const double v = (double)e[u];
+                if (isfinite(v)) s += v; else ++b;
+            }
+        }
     }
     ssum[threadIdx.x] = s;
     sbad[threadIdx.x] = b;
@@ -442,10 +457,14 @@ template <typename T>
 static hipError_t launch_reduce(const T* phi, int64_t n, double* part, unsigned long long* bad_part, double* sum,
                                 unsigned long long* bad, hipStream_t s)
 {
-    hipLaunchKernelGGL(reduce_partial_kernel<T>, dim3(kReduceBlocks), dim3(kThreads), 0, s, phi, n, part, bad_part);
+    // enough workgroups that every thread has at least 32 values (four batches) to add, up to
+    // kReduceBlocks (a 4096^2 field uses all of them: 32 waves per CU keep the loads in flight)
+    const int64_t want = (n + (int64_t)kThreads * 32 - 1) / ((int64_t)kThreads * 32);
+    const int blocks = (int)(want < 1 ? 1 : want > kReduceBlocks ? kReduceBlocks : want);
+    hipLaunchKernelGGL(reduce_partial_kernel<T>, dim3(blocks), dim3(kThreads), 0, s, phi, n, part, bad_part);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(reduce_final_kernel, dim3(1), dim3(kThreads), 0, s, part, bad_part, kReduceBlocks, sum, bad);
+    hipLaunchKernelGGL(reduce_final_kernel, dim3(1), dim3(kThreads), 0, s, part, bad_part, blocks, sum, bad);
     return hipGetLastError();
 }
 
