@@ -386,11 +386,12 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
         // and xi_y groups (rows [-4,TH+4), groups [0, TW/4)), four normals per call.
         constexpr int GXN = TW / 4 + 2, GYN = TW / 4;
         constexpr int NXI = GXN * TH, NYI = GYN * L::YH, NITEMS = NXI + NYI;
-        constexpr int PERX = (NXI + NT - 1) / NT, PERY = (NYI + NT - 1) / NT;
+        constexpr int PERX = (NXI + NT - 1) / NT;
         if (nx4 && interior) {
             // Items past the end of a list are skipped, not recomputed: a wave whose lanes
-            // are all past it branches over the call.  The xi_y items are dealt from thread
-            // 64 on, so the two lists' last, partial rounds fall on different waves.
+            // are all past it branches over the call.  The xi_y items go in vertical pairs,
+            // dealt from the thread after the last xi_x item (modulo the workgroup), so the
+            // two lists' last, partial rounds fall on different waves.
             // xi_x: rows [0, TH), column groups x0/4 - 1 .. x0/4 + TW/4
 #pragma unroll
             for (int u = 0; u < PERX; ++u) {
@@ -402,16 +403,41 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
                     emit_x4(r, 4 * gl - 4, z);
                 }
             }
-            // xi_y: rows [-4, TH+4), column groups x0/4 .. x0/4 + TW/4 - 1
-            const int ty = tid >= 64 ? tid - 64 : tid + NT - 64;
+            // xi_y in vertical pairs of items (rows r, r + 1 of one column group): the pair
+            // reads mu_p rows r-4 .. r+5 once (10 loads instead of 16)
+            constexpr int NYP = GYN * (L::YH / 2), PERYP = (NYP + NT - 1) / NT;
+            const int ty = (tid + NT - (NXI % NT)) % NT;
 #pragma unroll
-            for (int u = 0; u < PERY; ++u) {
+            for (int u = 0; u < PERYP; ++u) {
                 const int i = ty + u * NT;
-                if ((u + 1) * NT <= NYI || i < NYI) {
-                    const int rr = i / GYN, gl = i - rr * GYN;
-                    T z[4];
-                    normals4<T>((uint32_t)((x0 >> 2) + gl), (uint32_t)(y0 + rr - 4), step, 0, key0, key1, z);
-                    emit_y4(rr - 4, 4 * gl, z);
+                if ((u + 1) * NT <= NYP || i < NYP) {
+                    const int rp = i / GYN, gl = i - rp * GYN;
+                    const int r = 2 * rp - 4, c0 = 4 * gl;
+                    T zA[4], zB[4];
+                    normals4<T>((uint32_t)((x0 >> 2) + gl), (uint32_t)(y0 + r), step, 0, key0, key1, zA);
+                    normals4<T>((uint32_t)((x0 >> 2) + gl), (uint32_t)(y0 + r + 1), step, 0, key0, key1, zB);
+                    T accA[4], accB[4], dnB[4], up[4], nxt[4], dn[4];
+#pragma unroll
+                    for (int l = 0; l < 4; ++l) {
+                        accA[l] = K.amp * zA[l];
+                        accB[l] = K.amp * zB[l];
+                    }
+                    lds_load4(mup(r, c0), dnB);
+                    lds_load4(mup(r + 1, c0), up);
+#pragma unroll
+                    for (int j = 1; j <= 4; ++j) {
+                        lds_load4(mup(r + j + 1, c0), nxt);
+                        lds_load4(mup(r - j, c0), dn);
+#pragma unroll
+                        for (int l = 0; l < 4; ++l) {
+                            accA[l] = fma(-K.dyw[j], up[l] - dn[l], accA[l]);
+                            accB[l] = fma(-K.dyw[j], nxt[l] - dnB[l], accB[l]);
+                            up[l] = nxt[l];
+                            dnB[l] = dn[l];
+                        }
+                    }
+                    lds_store4(s_jy + (r + 4) * L::YW + c0, accA);
+                    lds_store4(s_jy + (r + 5) * L::YW + c0, accB);
                 }
             }
         } else if (nx4) {
