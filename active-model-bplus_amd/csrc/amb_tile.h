@@ -11,6 +11,10 @@ namespace amb {
 
 constexpr int kThreads = 256;
 
+// Fields of at least this many sites form the active current in the noise stage of FP32
+// 32x32 tiles (step_tile); smaller ones apply the 5x5 stencil of mu_a in stage B.
+constexpr long long kActiveCurrentSites = 2048LL * 2048;
+
 
 __device__ __forceinline__ int wrap_index(int g, int n)
 {
@@ -332,6 +336,7 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
     // rows [0,TH), cols [-4,TW+4); J_y on rows [-4,TH+4), cols [0,TW).  Stage B takes
     // their D8 divergence, the passive and the noise term in one.  Every path below
     // evaluates J in the same order (amp xi, then j = 1..4), so all give the same bits.
+    // FP32 noisy steps also form the active current C here (active_current below).
     bool noisy = NOISE == kNoisePhilox || NOISE == kNoiseBuffer;
     if constexpr (NOISE == kNoiseRuntime) noisy = K.amp != T(0);
     T* const s_jx = sm;  // aliases s_phi after stage A
@@ -380,6 +385,64 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
 #pragma unroll
         for (int j = 1; j <= 4; ++j) acc = fma(-K.dyw[j], *mup(r + j, col) - *mup(r - j, col), acc);
         return acc;
+    };
+    // The G3 current of the active terms, C = q - G3 mu_a (q the circulating current), on
+    // the box [-1, T+1), written over q; stage B then takes the G3 divergence of C, which
+    // equals -G3xoG3x mu_a - G3yoG3y mu_a + G3 . q, the 5x5 stencil of mu_a and the 3x3 of q
+    // that it applies otherwise.  Items are column segments of CSEG rows (each mu_a row read
+    // once, its x-difference and y-smoothed sum shared by the three rows that need them),
+    // dealt from the first wave after the Philox items.  It pays on large FP32 fields with
+    // the default 32x32 tile of 512 threads, whose 2-row stage-B strips form each row of
+    // mu_a work three times over: 6% fewer VALU instructions and, on MI355X, 102.1 -> 100.3
+    // us per noisy step at 4096^2 (the other tiles there lose 1-3%, but none is chosen).  At
+    // 1024^2 (one round of workgroups: 10.06 -> 10.23 us) and in FP64 (242 -> 250.5 us) it
+    // was slower, so those keep the 5x5 stencil.  The choice depends on the precision and
+    // the field's size only, so every tile shape and the batched kernels give the same bits.
+    constexpr bool kActiveCurrent = sizeof(T) == 4 && NOISE != kNoiseNone;
+    const bool active = kActiveCurrent && noisy && (long long)ny * nx >= kActiveCurrentSites;
+    auto active_current = [&]() {
+        if constexpr (kActiveCurrent) {
+            constexpr int CW = TW + 2, CH = TH + 2, CSEG = 4;
+            constexpr int NC = CW * ((CH + CSEG - 1) / CSEG);
+            constexpr int GXN = TW / 4 + 2, GYN = TW / 4;
+            constexpr int NXY = GXN * TH + GYN * (L::YH / 2);  // xi_x items and xi_y pairs
+            constexpr int C0 = ((NXY + 63) / 64 * 64) % NT;
+            for (int i = (tid + NT - C0) % NT; i < NC; i += NT) {
+                const int seg = i / CW, bc = i - seg * CW;
+                const int br0 = seg * CSEG;  // box row br is tile row br - 1, box column bc column bc - 1
+                const auto P = lds_base(s_mua + br0 * L::AW + bc);  // mu_a at (br0 - 2, bc - 2)
+                const auto Qx = lds_base(s_qx + (br0 + 1) * L::QW + bc + 1);
+                const auto Qy = lds_base(s_qy + (br0 + 1) * L::QW + bc + 1);
+                // every load ahead of the stores (the compiler cannot move an LDS read across
+                // a write it cannot prove disjoint, and would wait out each one in turn)
+                constexpr int NR = CSEG + 2;
+                T m[NR][3], qx[CSEG], qy[CSEG];
+#pragma unroll
+                for (int k = 0; k < NR; ++k)
+                    if (CH % CSEG == 0 || k < 2 || br0 + k - 2 < CH)
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) m[k][j] = P[k * L::AW + j];
+#pragma unroll
+                for (int k = 0; k < CSEG; ++k)
+                    if (CH % CSEG == 0 || br0 + k < CH) {
+                        qx[k] = Qx[k * L::QW];
+                        qy[k] = Qy[k * L::QW];
+                    }
+                T d[NR], sum[NR];  // x-difference and y-smoothed sum of each mu_a row
+#pragma unroll
+                for (int k = 0; k < NR; ++k)
+                    if (CH % CSEG == 0 || k < 2 || br0 + k - 2 < CH) {
+                        d[k] = m[k][2] - m[k][0];
+                        sum[k] = fma(K.gye, m[k][0] + m[k][2], K.gyc * m[k][1]);
+                    }
+#pragma unroll
+                for (int k = 0; k < CSEG; ++k)
+                    if (CH % CSEG == 0 || br0 + k < CH) {
+                        Qx[k * L::QW] = fma(-K.gxc, d[k + 1], fma(-K.gxe, d[k] + d[k + 2], qx[k]));
+                        Qy[k * L::QW] = qy[k] - (sum[k + 2] - sum[k]);
+                    }
+            }
+        }
     };
     if (NOISE == kNoisePhilox || (NOISE == kNoiseRuntime && noisy)) {
         // Philox items: xi_x groups (rows [0,TH), tile-relative column groups [-1, TW/4])
@@ -492,6 +555,7 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
                 }
             }
         }
+        if (active) active_current();
         __syncthreads();
     } else if (NOISE == kNoiseBuffer) {
         const size_t n = (size_t)ny * nx;
@@ -503,6 +567,7 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
             const int rr = i / L::YW, col = i - rr * L::YW;
             s_jy[i] = jy1(rr - 4, col, (T)noise[(size_t)wrap_index(y0 + rr - 4, ny) * nx + wrap_index(x0 + col, nx)]);
         }
+        if (active) active_current();
         __syncthreads();
     }
 
@@ -530,18 +595,21 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
     // noisy 32x32 tiles 97 VGPRs instead of 109, FP32 noisy 64x32 58 instead of 100, and
     // the FP32 noiseless 32x32 tile no longer spills; on MI355X the FP64 noiseless step at
     // 4096^2 went from 177 to 173 us and the FP32 noisy one at 1024^2 from 10.3 to 10.1 us).
-    T d2x[R + 4], sx[R + 4];  // mu_a rows r0-2 .. r0+R+1
-    T hx[R + 2], sy[R + 2];   // circulating current rows r0-1 .. r0+R
+    const bool lambda_b = !active;  // stage B applies G3xoG3x, G3yoG3y to mu_a
+    T d2x[R + 4], sx[R + 4];        // mu_a rows r0-2 .. r0+R+1
+    T hx[R + 2], sy[R + 2];         // rows r0-1 .. r0+R of q, or of C (above)
     T ycol[kNeverNoisy ? 1 : R + 8];  // J_y at rows r0-4 .. r0+R+3 of this column
     const auto Ab = lds_base(s_mua + r0 * L::AW + c + 2);
     const auto Qxb = lds_base(s_qx + (r0 + 1) * L::QW + c + 2);
     const auto Qyb = lds_base(s_qy + (r0 + 1) * L::QW + c + 2);
     const auto Yc = lds_base(s_jy + r0 * L::YW + c);
     auto row_a = [&](int k) {
-        const auto A = Ab + k * L::AW;
-        const T o2 = A[-2] + A[2];
-        d2x[k] = o2 - A[0] - A[0];                                            // (1,0,-2,0,1) along x
-        sx[k] = T(kG3G3S[0]) * A[0] + T(kG3G3S[1]) * (A[-1] + A[1]) + T(kG3G3S[2]) * o2;  // (.01,.06,.11,.06,.01)
+        if (lambda_b) {
+            const auto A = Ab + k * L::AW;
+            const T o2 = A[-2] + A[2];
+            d2x[k] = o2 - A[0] - A[0];                                            // (1,0,-2,0,1) along x
+            sx[k] = T(kG3G3S[0]) * A[0] + T(kG3G3S[1]) * (A[-1] + A[1]) + T(kG3G3S[2]) * o2;  // (.01,.06,.11,.06,.01)
+        }
     };
     auto row_q = [&](int k) {
         const auto Qx = Qxb + k * L::QW;
@@ -595,12 +663,15 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
                 }
             }
         }
-        // lambda: -(1/dx^2) G3xoG3x mu_a - (1/dy^2) G3yoG3y mu_a
-        acc = fma(K.lxx[0], d2x[k + 2], acc);
-        acc = fma(K.lxx[1], d2x[k + 1] + d2x[k + 3], acc);
-        acc = fma(K.lxx[2], d2x[k] + d2x[k + 4], acc);
-        acc = fma(K.lyy[0], (sx[k] + sx[k + 4]) - sx[k + 2] - sx[k + 2], acc);
-        // circulating: (1/dx) G3x q_x + (1/dy) G3y q_y
+        if (lambda_b) {
+            // lambda: -(1/dx^2) G3xoG3x mu_a - (1/dy^2) G3yoG3y mu_a
+            acc = fma(K.lxx[0], d2x[k + 2], acc);
+            acc = fma(K.lxx[1], d2x[k + 1] + d2x[k + 3], acc);
+            acc = fma(K.lxx[2], d2x[k] + d2x[k + 4], acc);
+            acc = fma(K.lyy[0], (sx[k] + sx[k + 4]) - sx[k + 2] - sx[k + 2], acc);
+        }
+        // circulating, (1/dx) G3x q_x + (1/dy) G3y q_y, or with the active current the
+        // whole active part, (1/dx) G3x C_x + (1/dy) G3y C_y
         acc = fma(K.gxc, hx[k + 1], acc);
         acc = fma(K.gxe, hx[k] + hx[k + 2], acc);
         acc += sy[k + 2] - sy[k];

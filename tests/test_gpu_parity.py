@@ -261,6 +261,31 @@ def test_tile_shapes_bit_identical(amb, temperature):
             assert np.array_equal(o, out[0])
 
 
+def test_active_current_fields_tiles_members_and_fp64(amb):
+    """FP32 noisy steps on fields of 2048^2 sites or more form the G3 current of the active
+    terms in the noise stage (smaller fields and FP64 apply the 5x5 stencil of mu_a in the
+    last stage): at 2048^2 every tile shape still gives the same bits, a batched member equals
+    the plain integrator, and the field stays within FP32 rounding of the FP64 path."""
+    st, md = (5e-3, 1, 1), (-0.25, 0, 0.25, 1, 0.5, -1, 0.05)
+    phi = 0.05 * np.random.default_rng(9).normal(size=(2048, 2048))
+    out = []
+    for t in range(len(amb.tile_shapes(32))):
+        sim = make(amb, phi, st, md, seed=5, precision=32, tile=t + 1)
+        sim.run(20)
+        out.append(sim.field)
+    for t, o in enumerate(out[1:], start=2):
+        assert np.array_equal(o, out[0]), amb.tile_shapes(32)[t - 1]
+    small = phi[:64, :96].copy()
+    ens = amb.Ensemble([phi, small], amb.Stencil(*st), [amb.Model(*md)] * 2, [5, 6], precision=32)
+    ens.run(20)
+    np.testing.assert_array_equal(ens.field(0), out[0])
+    ref = make(amb, phi, st, md, seed=5, precision=64)
+    ref.run(20)
+    err = relmax(out[0], ref.field)
+    print(f"2048^2 noisy, 20 steps: FP32 (active current) vs FP64 rel max-norm {err:.3e}")
+    assert err < 1e-5
+
+
 def test_default_tile_per_noise_mode(amb):
     """Without autotuning the tile comes from the occupancy rule over each kernel's order of
     preference: at 4096^2 FP32 the noisy kernel takes 32x32 tiles of 512 threads (four
