@@ -402,13 +402,23 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
     const bool active = kActiveCurrent && noisy && (long long)ny * nx >= kActiveCurrentSites;
     auto active_current = [&]() {
         if constexpr (kActiveCurrent) {
-            constexpr int CW = TW + 2, CH = TH + 2, CSEG = 4;
-            constexpr int NC = CW * ((CH + CSEG - 1) / CSEG);
+            constexpr int CW = TW + 2, CH = TH + 2, CSEG = 4, NSEG = (CH + CSEG - 1) / CSEG;
+            // the first CW32 columns of each segment go 32 to a half-wave, so that its 32
+            // lanes touch 32 consecutive words (one per LDS bank); the last columns follow
+            constexpr int CW32 = CW >= 32 ? CW / 32 * 32 : CW;
+            constexpr int NMAIN = CW32 * NSEG, NC = CW * NSEG;
             constexpr int GXN = TW / 4 + 2, GYN = TW / 4;
             constexpr int NXY = GXN * TH + GYN * (L::YH / 2);  // xi_x items and xi_y pairs
             constexpr int C0 = ((NXY + 63) / 64 * 64) % NT;
             for (int i = (tid + NT - C0) % NT; i < NC; i += NT) {
-                const int seg = i / CW, bc = i - seg * CW;
+                int seg, bc;
+                if (i < NMAIN) {
+                    seg = i / CW32;
+                    bc = i - seg * CW32;
+                } else {
+                    seg = (i - NMAIN) / (CW - CW32);
+                    bc = CW32 + (i - NMAIN) - seg * (CW - CW32);
+                }
                 const int br0 = seg * CSEG;  // box row br is tile row br - 1, box column bc column bc - 1
                 const auto P = lds_base(s_mua + br0 * L::AW + bc);  // mu_a at (br0 - 2, bc - 2)
                 const auto Qx = lds_base(s_qx + (br0 + 1) * L::QW + bc + 1);
