@@ -393,11 +393,12 @@ __device__ __forceinline__ void step_tile(T* const sm, const T* __restrict__ in,
     // once, its x-difference and y-smoothed sum shared by the three rows that need them),
     // dealt from the first wave after the Philox items.  It pays on large FP32 fields with
     // the default 32x32 tile of 512 threads, whose 2-row stage-B strips form each row of
-    // mu_a work three times over: 6% fewer VALU instructions and, on MI355X, 102.1 -> 100.3
-    // us per noisy step at 4096^2 (the other tiles there lose 1-3%, but none is chosen).  At
-    // 1024^2 (one round of workgroups: 10.06 -> 10.23 us) and in FP64 (242 -> 250.5 us) it
-    // was slower, so those keep the 5x5 stencil.  The choice depends on the precision and
-    // the field's size only, so every tile shape and the batched kernels give the same bits.
+    // mu_a work three times over: about 5% fewer VALU instructions and, on MI355X, 103.7 ->
+    // 101.8 us per noisy step at 4096^2 (3072^2 -1.6%, 2048^2 -1.1%, but 1536^2 +0.8%; the
+    // other tiles lose 1-3% at 4096^2, but none of them is chosen there).  In FP64 it was
+    // slower (242 -> 250.5 us), so FP64 and the smaller FP32 fields keep the 5x5 stencil.
+    // The choice depends on the precision and the field's size only, so every tile shape
+    // and the batched kernels give the same bits.
     constexpr bool kActiveCurrent = sizeof(T) == 4 && NOISE != kNoiseNone;
     const bool active = kActiveCurrent && noisy && (long long)ny * nx >= kActiveCurrentSites;
     auto active_current = [&]() {
