@@ -54,12 +54,13 @@ struct TileShape {
 // fastest at 4096^2 (108.1 us per step; 32x48 111.5 us, 32x16 116.9 us, the others 120-139
 // us): their 35 KB of LDS and 61 VGPRs let four workgroups, 8 waves per SIMD, share a CU,
 // against three for 32x48, and the extra halo work costs less than the extra waves hide;
-// at 1024^2 they tie with 64x32 (11.02 / 11.04 us).  Noiseless: 32x48 is fastest at 4096^2
-// (90.6 us; 32x32x512 93.1 us, whose 17-tap windows spill at 64 VGPRs; 32x24 96.0 us) and
-// 64x32 at 1024^2 (9.2 us, 32x32 9.5 us).  16x8 serves the smallest fields (256^2: 5.1 us
-// noiseless, 6.2 us noisy, 8x8 6.0 us).
+// at 1024^2 they tie with 64x32 (11.02 / 11.04 us).  Noiseless, measured with the current
+// step (stage B's rows formed just before use, so its 17-tap windows fit 58 VGPRs without
+// spilling): 32x32 tiles of 512 threads are fastest at 4096^2 (84.9 us; 32x48 86.3 us, 32x16
+// 93.4 us), 2048^2 (25.0 / 25.6 us) and 1024^2 (8.92 us, 64x32 8.94 us, 32x48 9.79 us).
+// 16x8 serves the smallest fields (256^2: 5.1 us noiseless, 6.2 us noisy, 8x8 6.0 us).
 #define AMB_F32_NOISY_ORDER 9, 0, 3, 2, 1, 4, 5, 6, 7, 8
-#define AMB_F32_NOISELESS_ORDER 0, 3, 9, 1, 5, 2, 4, 6, 7, 8
+#define AMB_F32_NOISELESS_ORDER 9, 0, 3, 1, 5, 2, 4, 6, 7, 8
 // FP64, noisy and noiseless alike: 32x32 tiles of 512 threads (held to 128 VGPRs, so two
 // workgroups, 16 waves, share a CU) are fastest at 4096^2 (291.0 / 178.6 us per step against
 // 321.0 / 213.2 us for 256 threads, both before the fitted noise polynomials and the priorities); 16x8 serves the smallest fields (256^2: 5.56 us, 16x16

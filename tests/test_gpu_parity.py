@@ -307,8 +307,8 @@ def test_default_tile_per_noise_mode(amb):
 
 
 def _check_default_tiles(amb):
-    for precision, side, temperature, want in [(32, 4096, 0.05, (32, 32, 512)), (32, 4096, 0.0, (32, 48, 512)),
-                                               (32, 1024, 0.1, (32, 32, 512)), (32, 1024, 0.0, (64, 32, 512)),
+    for precision, side, temperature, want in [(32, 4096, 0.05, (32, 32, 512)), (32, 4096, 0.0, (32, 32, 512)),
+                                               (32, 1024, 0.1, (32, 32, 512)), (32, 1024, 0.0, (32, 32, 512)),
                                                (32, 512, 0.05, (32, 32, 512)), (32, 512, 0.0, (32, 32, 512)),
                                                (32, 256, 0.0, (16, 8, 128)), (32, 256, 0.05, (16, 8, 128)),
                                                (64, 256, 0.0, (16, 8, 128)), (64, 512, 0.05, (32, 32, 512)),
