@@ -4,16 +4,19 @@
 // (8 B per site in FP32, 16 B in FP64).  The reference needs three launches per
 // step and ~280 B per site (src/integrator.cu:46-262, host loop :507-518).
 //
-// Per tile of TW x TH outputs (one 256-thread workgroup, four wave64s):
+// Per tile of TW x TH outputs (one workgroup of NT threads; 32x32 tiles of 512 threads,
+// eight wave64s, on large fields):
 //   stage 0  phi on rows [-9, TH+9) x cols [-9, TW+9) -> LDS (periodic wrap)
 //   stage A  on the cross-shaped halo region: G3x phi, G3y phi, L3 phi, then
-//            mu_p (rows/cols +-8), mu_a = lambda|G3 phi|^2 (box +-2) and the
-//            circulating current zeta L3phi G3phi (box +-1) -> LDS
-//   stage N  conserved-noise normals xi_x (cols +-4), xi_y (rows +-4) from
-//            counter-based Philox4x32-10 -> LDS (aliases the phi buffer)
-//   stage B  div J per output from LDS: D8oD8 on mu_p (17 taps per axis),
-//            G3oG3 on mu_a (5x5 separable), G3 on the circulating current, D8 on
-//            the noise; phi^{n+1} = phi^n - dt div J.
+//            mu_p (rows/cols +-8), mu_a = lambda|G3 phi|^2 and the circulating current
+//            q = zeta L3phi G3phi (box +-2) -> LDS
+//   stage N  (noisy steps) conserved-noise normals xi_x (cols +-4), xi_y (rows +-4) from
+//            counter-based Philox4x32-10, each turned at once into the current
+//            J = amp xi - D8 mu_p (aliases the phi buffer); on FP32 fields of 2048^2
+//            sites or more also the active current C = q - G3 mu_a (box +-1, over q)
+//   stage B  div J per output from LDS: the D8 divergence of J (noisy) or D8oD8 on mu_p
+//            (17 taps per axis, noiseless), G3oG3 on mu_a (5x5 separable) plus G3 on q,
+//            or G3 on C alone; phi^{n+1} = phi^n - dt div J.
 // A site's noise depends only on (seed, step, row, column, component), so the
 // halo copies that neighbouring tiles generate are bit-identical to the owner's.
 #include <hip/hip_runtime.h>
