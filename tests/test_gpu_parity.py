@@ -284,6 +284,17 @@ def test_active_current_fields_tiles_members_and_fp64(amb):
     err = relmax(out[0], ref.field)
     print(f"2048^2 noisy, 20 steps: FP32 (active current) vs FP64 rel max-norm {err:.3e}")
     assert err < 1e-5
+    # the shared-buffer noise mode takes the same path
+    buf = np.random.default_rng(10).normal(size=(3, 2) + phi.shape)
+    runs = []
+    for precision in (32, 64):
+        sim = make(amb, phi, st, md, seed=5, precision=precision)
+        sim.set_noise(buf)
+        sim.run(3)
+        runs.append(sim.field)
+    err = relmax(runs[0], runs[1])
+    print(f"2048^2 shared noise buffer, 3 steps: FP32 (active current) vs FP64 rel max-norm {err:.3e}")
+    assert err < 1e-5
 
 
 def test_default_tile_per_noise_mode(amb):
