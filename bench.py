@@ -369,7 +369,8 @@ def main():
     p = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=None, help="timed steps (default 2000; c5: 20000 coarse steps)")
-    p.add_argument("--warmup", type=int, default=64)
+    p.add_argument("--warmup", type=int, default=300,
+                   help="untimed steps first (the GPU clock ramps up over a process's first ~300 step launches)")
     p.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c4", "c5", "copy"])
     p.add_argument("--audit-every", type=int, default=10, help="c5: initial coarse steps per Euler audit")
     p.add_argument("--max-audit-every", type=int, default=200, help="c5: longest audit interval")
