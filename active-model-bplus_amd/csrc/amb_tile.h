@@ -11,10 +11,9 @@ namespace amb {
 
 constexpr int kThreads = 256;
 
-// Fields of at least this many sites form the active current in the noise stage of FP32
-// 32x32 tiles (step_tile); smaller ones apply the 5x5 stencil of mu_a in stage B.
+// Noisy FP32 steps on fields of at least this many sites form the active current in the
+// noise stage (step_tile); smaller fields and FP64 apply the 5x5 stencil of mu_a in stage B.
 constexpr long long kActiveCurrentSites = 2048LL * 2048;
-
 
 __device__ __forceinline__ int wrap_index(int g, int n)
 {
