@@ -295,6 +295,19 @@ def test_active_current_fields_tiles_members_and_fp64(amb):
     err = relmax(runs[0], runs[1])
     print(f"2048^2 shared noise buffer, 3 steps: FP32 (active current) vs FP64 rel max-norm {err:.3e}")
     assert err < 1e-5
+    # a field past the size whose columns do not split into groups of four (every tile takes
+    # the per-site noise path)
+    odd = 0.05 * np.random.default_rng(11).normal(size=(2050, 2050))
+    shapes = amb.tile_shapes(32)
+    runs = []
+    for t, precision in ((1, 32), (len(shapes), 32), (0, 64)):
+        sim = make(amb, odd, st, md, seed=7, precision=precision, tile=t)
+        sim.run(10)
+        runs.append(sim.field)
+    assert np.array_equal(runs[0], runs[1])
+    err = relmax(runs[0], runs[2])
+    print(f"2050^2 noisy, 10 steps: FP32 (active current) vs FP64 rel max-norm {err:.3e}")
+    assert err < 1e-5
 
 
 def test_default_tile_per_noise_mode(amb):
